@@ -1,0 +1,23 @@
+"""geothermalwells.jl_amd -- MI355X (gfx950) implementation of GeothermalWells.jl's
+per-timestep hot path (z-diffusion stencil under ROCK2, ADI x/y Thomas solves,
+semi-Lagrangian pipe advection, outlet reductions) behind the reference's API.
+
+The compute path is libgeowells_hip.so (HIP kernels + C ABI, include/geowells.h);
+this package is the host-side mirror of the reference's Julia interface.
+"""
+from . import _abi
+from .cache import (Cache, CallbackSet, DiscreteCallback, SavedValues, create_batch_cache, create_cache,
+                    get_simulation_callback)
+from .grids import compute_domain, create_adaptive_grid_1d, create_uniform_gridz_with_borehole_depths
+from .physics import (AbstractBorehole, AbstractMaterialProperties, Borehole, ConstantInlet, CustomInlet,
+                      HeatExchangerInlet, HomogenousMaterialProperties, PerWellInlet, StratifiedMaterialProperties,
+                      initial_condition_thermal_gradient)
+from .solver import (ADI_and_ADV_callback_, ODEProblem, ROCK2, eigen_estimator, rhs_diffusion_z_,
+                     rock2_coefficients, solve)
+
+LIB_PATH = _abi.LIB_PATH
+
+
+def load_library():
+    """Load libgeowells_hip.so (raises if it is not built)."""
+    return _abi.lib()

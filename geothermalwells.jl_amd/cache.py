@@ -1,0 +1,296 @@
+"""create_cache / get_simulation_callback of GeothermalWells.jl (src/cache.jl),
+backed by a libgeowells_hip context."""
+import ctypes
+import os
+
+import numpy as np
+import torch
+
+from . import _abi
+from .physics import (ConstantInlet, CustomInlet, HeatExchangerInlet, HomogenousMaterialProperties, PerWellInlet,
+                      StratifiedMaterialProperties)
+
+__all__ = ["create_cache", "create_batch_cache", "Cache", "get_simulation_callback", "DiscreteCallback",
+           "CallbackSet", "SavedValues"]
+
+DEFAULT_ROCK2_C3 = 1.0 / 16.0
+
+
+def _dptr(arr):
+    return arr.ctypes.data_as(_abi.c_double_p)
+
+
+class _Scenario:
+    """Keeps the host arrays of one gw_scenario alive."""
+
+    def __init__(self, gridx, gridy, gridz, materials, boreholes, inlet_model):
+        self.gridx = np.ascontiguousarray(np.asarray(gridx, dtype=np.float64))
+        self.gridy = np.ascontiguousarray(np.asarray(gridy, dtype=np.float64))
+        self.gridz = np.ascontiguousarray(np.asarray(gridz, dtype=np.float64))
+        self.boreholes = tuple(boreholes)
+        self.materials = materials
+        self.inlet_model = inlet_model
+        nb = len(self.boreholes)
+        self.bh_arr = (_abi.GwBorehole * nb)(*[_abi.GwBorehole(*b.args) for b in self.boreholes])
+        m = _abi.GwMaterials()
+        if isinstance(materials, StratifiedMaterialProperties):
+            self.layers = [np.ascontiguousarray(np.array(v, dtype=np.float64)) for v in
+                           (materials.k_rock_layers, materials.rho_c_rock_layers, materials.layer_depths)]
+            m.n_layers = len(materials.layer_depths)
+            m.k_rock_layers, m.rho_c_rock_layers, m.layer_depths = [_dptr(a) for a in self.layers]
+            m.k_rock, m.rho_c_rock = materials.k_rock_layers[0], materials.rho_c_rock_layers[0]
+        elif isinstance(materials, HomogenousMaterialProperties):
+            m.n_layers = 0
+            m.k_rock, m.rho_c_rock = materials.k_rock, materials.rho_c_rock
+        else:
+            raise TypeError("materials must be HomogenousMaterialProperties or StratifiedMaterialProperties")
+        for f in ("k_water", "rho_c_water", "k_steel", "rho_c_steel", "k_insulating", "rho_c_insulating",
+                  "k_backfill", "rho_c_backfill"):
+            setattr(m, f, getattr(materials, f))
+        inl = _abi.GwInlet()
+        self._cb = None
+        if isinstance(inlet_model, ConstantInlet):
+            inl.kind, inl.value = _abi.GW_INLET_CONSTANT, inlet_model.T_const
+        elif isinstance(inlet_model, HeatExchangerInlet):
+            inl.kind, inl.value = _abi.GW_INLET_HEAT_EXCHANGER, inlet_model.ΔT
+        elif isinstance(inlet_model, PerWellInlet):
+            if len(inlet_model.values) != nb:
+                raise ValueError("PerWellInlet needs one value per borehole")
+            self.per_well = np.ascontiguousarray(np.array(inlet_model.values, dtype=np.float64))
+            inl.kind, inl.per_well = _abi.GW_INLET_PER_WELL, _dptr(self.per_well)
+        elif isinstance(inlet_model, CustomInlet) or callable(inlet_model):
+            f = inlet_model
+
+            def _host_inlet(bh, T_out, n_bh, t, user):
+                vals = [T_out[q] for q in range(n_bh)]
+                return float(f(int(bh), vals, float(t)))
+
+            self._cb = _abi.INLET_FN(_host_inlet)
+            inl.kind, inl.fn = _abi.GW_INLET_HOST, self._cb
+        else:
+            raise TypeError("unknown inlet model")
+        sc = _abi.GwScenario()
+        sc.nx, sc.ny, sc.nz = len(self.gridx), len(self.gridy), len(self.gridz)
+        sc.gridx, sc.gridy, sc.gridz = _dptr(self.gridx), _dptr(self.gridy), _dptr(self.gridz)
+        sc.n_bh = nb
+        sc.boreholes = ctypes.cast(self.bh_arr, ctypes.POINTER(_abi.GwBorehole))
+        sc.materials = m
+        sc.inlet = inl
+        self.c = sc
+
+    @property
+    def shape(self):
+        return (len(self.gridx), len(self.gridy), len(self.gridz))
+
+
+class Cache:
+    """The `cache` (the ODEProblem parameter `p`) of one or more scenarios.
+
+    Fields follow the reference NamedTuple (src/cache.jl:83-113) where they
+    have a host meaning; `T_outlet` reads the device outlet of the last
+    advection sub-step."""
+
+    def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3):
+        self._scen = scenarios
+        self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
+        if self.backend.type != "cuda":
+            raise ValueError("libgeowells_hip runs on a GPU device only")
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "f64" else torch.float32
+        self.rock2_c3 = float(rock2_c3)
+        lib = _abi.lib()
+        arr = (_abi.GwScenario * len(scenarios))(*[s.c for s in scenarios])
+        self._arr = arr
+        prob = _abi.GwProblem()
+        prob.n_scen = len(scenarios)
+        prob.scen = ctypes.cast(arr, ctypes.POINTER(_abi.GwScenario))
+        prob.precision = _abi.GW_F64 if precision == "f64" else _abi.GW_F32_STATE
+        prob.stream = stream
+        prob.rock2_c3 = self.rock2_c3
+        prob.flags = 0
+        h = ctypes.c_void_p()
+        _abi.check(lib.gw_create(ctypes.byref(prob), ctypes.byref(h)))
+        self._ctx = h
+        self._lib = lib
+        s0 = scenarios[0]
+        self.gridx, self.gridy, self.gridz = s0.gridx, s0.gridy, s0.gridz
+        self.Nx, self.Ny, self.Nz = s0.shape
+        self.N_bh = len(s0.boreholes)
+        self.materials = s0.materials
+        self.boreholes = s0.boreholes
+        self.inlet_model = s0.inlet_model
+        cz, ni, no = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _abi.check(lib.gw_counts(h, 0, ctypes.byref(cz), ctypes.byref(ni), ctypes.byref(no)))
+        self.countz, self.countxy_inner, self.countxy_outer = cz.value, ni.value, no.value
+        self.eigen_estimate = lib.gw_eigen_estimate(h, 0)
+        self.n_bh_total = sum(len(s.boreholes) for s in scenarios)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_ctx", None) and self._ctx.value:
+                self._lib.gw_destroy(self._ctx)
+                self._ctx = ctypes.c_void_p()
+        except Exception:
+            pass
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    @property
+    def n_scenarios(self):
+        return len(self._scen)
+
+    def scenario_shape(self, s):
+        return self._scen[s].shape
+
+    def scenario_offset(self, s):
+        return int(self._lib.gw_scenario_offset(self._ctx, s))
+
+    @property
+    def state_elems(self):
+        return int(self._lib.gw_state_elems(self._ctx))
+
+    def index_lists(self, s=0):
+        """(Idx_list_Inner, Idx_list_Outer) as 1-based (i, j, bh) tuples (src/cache.jl:17-30)."""
+        cz, ni, no = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _abi.check(self._lib.gw_counts(self._ctx, s, ctypes.byref(cz), ctypes.byref(ni), ctypes.byref(no)))
+        inner = (ctypes.c_int32 * max(3 * ni.value, 1))()
+        outer = (ctypes.c_int32 * max(3 * no.value, 1))()
+        _abi.check(self._lib.gw_index_lists(self._ctx, s, inner, outer))
+        to = lambda a, n: [(a[3 * q] + 1, a[3 * q + 1] + 1, a[3 * q + 2] + 1) for q in range(n)]
+        return to(inner, ni.value), to(outer, no.value)
+
+    @property
+    def Idx_list_Inner(self):
+        return self.index_lists(0)[0]
+
+    @property
+    def Idx_list_Outer(self):
+        return self.index_lists(0)[1]
+
+    @property
+    def T_outlet(self):
+        out = np.zeros(self.n_bh_total, dtype=np.float64)
+        _abi.check(self._lib.gw_outlet(self._ctx, _dptr(out)))
+        return out
+
+    def set_inlet(self, values):
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64))
+        _abi.check(self._lib.gw_set_inlet(self._ctx, _dptr(v)))
+
+    def check_state(self, phi):
+        if not isinstance(phi, torch.Tensor) or phi.device.type != "cuda":
+            raise TypeError("the state must be a CUDA/HIP tensor")
+        if phi.dtype != self.dtype or not phi.is_contiguous() or phi.numel() != self.state_elems:
+            raise ValueError(f"state must be a contiguous {self.dtype} tensor with {self.state_elems} elements")
+        return ctypes.c_void_p(phi.data_ptr())
+
+    def new_state(self):
+        return torch.empty(self.state_elems, dtype=self.dtype, device=self.backend)
+
+    def scenario_view(self, phi, s):
+        nx, ny, nz = self.scenario_shape(s)
+        off = self.scenario_offset(s)
+        return phi.view(-1)[off:off + nx * ny * nz].view(nx, ny, nz)
+
+    def sync(self):
+        _abi.check(self._lib.gw_sync(self._ctx))
+
+
+def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inlet_model, precision="f64",
+                 stream=None, rock2_c3=DEFAULT_ROCK2_C3):
+    """create_cache (src/cache.jl:55-116)."""
+    return Cache([_Scenario(gridx, gridy, gridz, materials, boreholes, inlet_model)], backend=backend,
+                 precision=precision, stream=stream, rock2_c3=rock2_c3)
+
+
+def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3):
+    """Many independent scenarios (each a dict with the create_cache keywords
+    gridx, gridy, gridz, materials, boreholes, inlet_model) advanced together:
+    every kernel launch covers all of them."""
+    sc = [_Scenario(d["gridx"], d["gridy"], d["gridz"], d["materials"], d["boreholes"], d["inlet_model"])
+          for d in scenarios]
+    return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3)
+
+
+# ---------------------------------------------------------------------------
+# Callbacks (src/cache.jl:132-237), with the DiffEqBase/DiffEqCallbacks
+# semantics the reference relies on.
+# ---------------------------------------------------------------------------
+class DiscreteCallback:
+    def __init__(self, condition, affect, save_positions=(False, False)):
+        self.condition = condition
+        self.affect = affect
+        self.save_positions = save_positions
+
+
+class CallbackSet:
+    def __init__(self, *callbacks):
+        self.callbacks = list(callbacks)
+
+
+class SavedValues:
+    def __init__(self, dtype=np.float32):
+        self.t = []
+        self.saveval = []
+        self.dtype = dtype
+
+
+class SavingCallback:
+    def __init__(self, save_func, saved_values, saveat):
+        self.save_func = save_func
+        self.saved_values = saved_values
+        self.saveat = [float(s) for s in saveat]
+
+    def maybe_save(self, integrator, tol=1e-9):
+        t = integrator.t
+        for s in self.saveat:
+            if abs(s - t) <= tol * max(1.0, abs(s)):
+                self.saved_values.t.append(float(t))
+                self.saved_values.saveval.append(self.save_func(integrator.u, t, integrator))
+                break
+
+
+def save_and_print_callback(saveat, print_every_n=1000, write_to_jld=False, data_folder_dir="",
+                            Float_used_to_save=np.float32):
+    """src/cache.jl:132-184: progress printing and host snapshots (Float32 by default)."""
+
+    def print_affect(integrator):
+        t = integrator.t
+        n = integrator.stats.naccept
+        if t > 3600 * 24 * 365:
+            print(f"Step {n}, t = {t} s, or {round(t / 31536000, 4)} years", flush=True)
+        elif t > 3600 * 24:
+            print(f"Step {n}, t = {t} s, or {round(t / 86400, 2)} days", flush=True)
+        else:
+            print(f"Step {n}, t = {t} s, or {round(t / 3600, 2)} hours", flush=True)
+
+    print_cb = DiscreteCallback(lambda u, t, integ: integ.stats.naccept % print_every_n == 0, print_affect)
+    saved_values = SavedValues(Float_used_to_save)
+    counter = [0]
+
+    def save_host(u, t, integrator):
+        u_cpu = u.detach().to("cpu").numpy().astype(Float_used_to_save, copy=True)
+        if write_to_jld:
+            counter[0] += 1
+            np.save(os.path.join(data_folder_dir, f"checkpoint_{counter[0]}.npy"), u_cpu)
+        return u_cpu
+
+    if write_to_jld and not os.path.isdir(data_folder_dir):
+        os.mkdir(data_folder_dir)
+    save_cb = SavingCallback(save_host, saved_values, saveat)
+    return save_cb, print_cb, saved_values
+
+
+def get_simulation_callback(*, saveat, print_every_n=1000, write_to_jld=False, data_folder_dir=""):
+    """src/cache.jl:224-237: (CallbackSet(ADI_and_ADV, print_cb, save_cb), saved_values)."""
+    from .solver import ADI_and_ADV_callback_
+    if write_to_jld and not data_folder_dir:
+        raise RuntimeError("When `write_to_jld` is true, `data_folder_dir` must be provided and cannot be an "
+                           "empty string")
+    save_cb, print_cb, saved_values = save_and_print_callback(saveat, print_every_n=print_every_n,
+                                                              write_to_jld=write_to_jld,
+                                                              data_folder_dir=data_folder_dir)
+    adi = DiscreteCallback(lambda u, t, integ: True, ADI_and_ADV_callback_)
+    return CallbackSet(adi, print_cb, save_cb), saved_values

@@ -1,0 +1,985 @@
+// gw_host.cpp -- set-up and C ABI of libgeowells_hip.
+//
+// Host work done once per context (create_cache, src/cache.jl:55-116):
+//   * Borehole derived fields exactly as the reference constructor forms them
+//     (src/physics.jl:48-60);
+//   * material classification of every (i, j) column with the reference's
+//     point-wise rules (src/physics.jl:199-274), evaluated in FP64 without
+//     contraction so every branch decision matches; columns with identical
+//     k(z), rho*c(z) profiles share a class, levels with identical values in
+//     all classes share a zone;
+//   * coefficients of the 1-D operators (src/solver.jl:34-109, 143-205) per
+//     class and level (z) and per distinct (line, zone) (x and y);
+//   * the pipe index lists in the reference's order (src/cache.jl:9-44).
+// Work done once per sub-step length c = dt/2 (the reference's dt is fixed):
+//   * LU factors of (I - cA) for every line set (Thomas without pivoting,
+//     src/solver.jl:207-218);
+//   * semi-Lagrangian departure tables per (borehole, level), with the
+//     reference's branch arithmetic (src/solver.jl:299-376), and the level
+//     ranges of the turnaround means and the outlet.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "geowells.h"
+#include "gw_kernels.hpp"
+
+using namespace gw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+struct HipFail {
+  hipError_t e;
+};
+inline void hip_check(hipError_t e) {
+  if (e != hipSuccess) throw HipFail{e};
+}
+
+struct HostBh {
+  double xc, yc, h, r_inner, t_inner, r_and_t_inner_sq, r_outer, r_outer_sq, t_outer,
+      r_and_t_outer_sq, r_backfill, r_backfill_sq, mdot, v_inner, v_outer, insulation_depth;
+};
+
+// Borehole{T}(...)  (src/physics.jl:48-60)
+HostBh make_bh(const gw_borehole& a) {
+  const double rho_water = 998.2;
+  const double pi = 3.14159265358979323846;
+  HostBh b;
+  b.xc = a.xc; b.yc = a.yc; b.h = a.h; b.r_inner = a.r_inner; b.t_inner = a.t_inner;
+  b.r_and_t_inner_sq = (a.r_inner + a.t_inner) * (a.r_inner + a.t_inner);
+  b.r_outer = a.r_outer;
+  b.r_outer_sq = a.r_outer * a.r_outer;
+  b.t_outer = a.t_outer;
+  b.r_and_t_outer_sq = (a.r_outer + a.t_outer) * (a.r_outer + a.t_outer);
+  b.r_backfill = a.r_backfill;
+  b.r_backfill_sq = a.r_backfill * a.r_backfill;
+  b.mdot = a.mdot;
+  const double A_inner = pi * (a.r_inner * a.r_inner);
+  const double A_outer = pi * b.r_outer_sq - pi * b.r_and_t_inner_sq;
+  b.v_inner = a.mdot / (A_inner * rho_water);
+  b.v_outer = a.mdot / (A_outer * rho_water);
+  b.insulation_depth = a.insulation_depth;
+  return b;
+}
+
+struct Mat {
+  std::vector<double> kl, rl, depth;  // layers (empty: homogeneous)
+  double k_rock, rc_rock, k_water, rc_water, k_steel, rc_steel, k_ins, rc_ins, k_bf, rc_bf;
+  double rock_k(double z) const {
+    if (depth.empty()) return k_rock;
+    for (size_t l = 0; l < depth.size(); ++l) if (z <= depth[l]) return kl[l];
+    return kl.back();
+  }
+  double rock_rc(double z) const {
+    if (depth.empty()) return rc_rock;
+    for (size_t l = 0; l < depth.size(); ++l) if (z <= depth[l]) return rl[l];
+    return rl.back();
+  }
+  // eigen_estimator_get_dmax (src/physics.jl:134-149, 177-185)
+  double dmax() const {
+    double d = 0;
+    if (depth.empty()) d = k_rock / rc_rock;
+    else for (size_t l = 0; l < kl.size(); ++l) d = std::max(d, kl[l] / rl[l]);
+    d = std::max(d, k_water / rc_water);
+    d = std::max(d, k_steel / rc_steel);
+    d = std::max(d, k_ins / rc_ins);
+    d = std::max(d, k_bf / rc_bf);
+    return d;
+  }
+};
+
+// get_thermal_conductivity / get_volumetric_heat_capacity at one point,
+// restricted to the boreholes `near` (all others are skipped by the
+// reference's radius test anyway).
+void point_material(double x, double y, double z, const std::vector<HostBh>& bhs,
+                    const std::vector<int>& near, const Mat& m, double& k, double& rc) {
+  k = m.rock_k(z);
+  rc = m.rock_rc(z);
+  for (int bi : near) {
+    const HostBh& bh = bhs[bi];
+    const double dx = x - bh.xc, dy = y - bh.yc;
+    const double r_sq = dx * dx + dy * dy;
+    if (!(r_sq > bh.r_backfill_sq)) {  // src/physics.jl:207-228
+      if (z <= bh.h) {
+        if (r_sq < bh.r_inner * bh.r_inner) k = m.k_water;
+        else if (r_sq < bh.r_and_t_inner_sq) k = m.k_ins;
+        else if (r_sq < bh.r_outer_sq) k = m.k_water;
+        else if (r_sq < bh.r_and_t_outer_sq) k = (z <= bh.insulation_depth) ? m.k_ins : m.k_steel;
+        else k = m.k_bf;
+      } else if (z <= bh.h + 3) {
+        k = m.k_steel;
+      }
+    }
+    if (!(r_sq > bh.r_and_t_outer_sq)) {  // src/physics.jl:251-270
+      if (z <= bh.h) {
+        if (r_sq < bh.r_inner * bh.r_inner) rc = m.rc_water;
+        else if (r_sq < bh.r_and_t_inner_sq) rc = m.rc_ins;
+        else if (r_sq < bh.r_outer_sq) rc = m.rc_water;
+        else rc = (z <= bh.insulation_depth) ? m.rc_ins : m.rc_steel;
+      } else if (z <= bh.h + 3) {
+        rc = m.rc_steel;
+      }
+    }
+  }
+}
+
+// Explicit operator coefficients along one line (src/solver.jl:34-109):
+// (A u)_l = ea_l (u_{l+1} - u_l) - eb_l (u_l - u_{l-1}), zero-flux ends.
+void line_coefficients(const double* g, int N, const double* kc, const double* rc, double* ea,
+                       double* eb) {
+  for (int l = 0; l < N; ++l) {
+    const double dp = (l == N - 1) ? (g[l] - g[l - 1]) : (g[l + 1] - g[l]);
+    const double dm = (l == 0) ? dp : (g[l] - g[l - 1]);
+    const double kp = (l == N - 1) ? kc[l] : kc[l + 1];
+    const double km = (l == 0) ? kc[l] : kc[l - 1];
+    const double den = (dp + dm) * 0.5;
+    ea[l] = (l == N - 1) ? 0.0 : ((kp + kc[l]) * 0.5 / dp) / den / rc[l];
+    eb[l] = (l == 0) ? 0.0 : ((kc[l] + km) * 0.5 / dm) / den / rc[l];
+  }
+}
+
+struct Scen {
+  int nx, ny, nz;
+  std::vector<double> gx, gy, gz;
+  std::vector<HostBh> bh;
+  Mat mat;
+  gw_inlet inlet;
+  std::vector<double> per_well;
+  // classification
+  std::vector<uint16_t> cls;      // nx*ny, index j + ny*i
+  int ncls = 0;
+  std::vector<double> kprof, rprof;  // ncls*nz
+  std::vector<int32_t> zone;      // nz
+  int nzone = 0;
+  std::vector<int> zone_rep;      // representative level of each zone
+  std::vector<double> zcoef;      // ncls*nz*2
+  std::vector<int32_t> xset, yset;  // ny*nzone, nx*nzone (global set ids)
+  // pipes (reference order)
+  std::vector<std::array<int, 3>> idx_inner, idx_outer;
+  int countz = 0;
+  double eigen = 0;
+  double zrho = 0;                // Gershgorin bound of the spectral radius of A_z
+  int64_t off = 0;
+};
+
+struct LineSet {
+  int N;
+  std::vector<double> ea, eb;
+};
+
+struct CTab {
+  double c = 0;
+  std::vector<BhDev> h_bh;
+  int64_t* d_set_off = nullptr;
+  double* d_coef = nullptr;
+  BhDev* d_bh = nullptr;
+  Dep* d_dep = nullptr;
+  double* d_means = nullptr;
+  int nmeans = 0;
+};
+
+template <class T>
+T* dev_upload(const std::vector<T>& h) {
+  T* d = nullptr;
+  const size_t n = std::max<size_t>(h.size(), 1) * sizeof(T);
+  hip_check(hipMalloc(&d, n));
+  if (!h.empty()) hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+}  // namespace
+
+struct gw_ctx {
+  int prec = GW_F64;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  double c3 = 1.0 / 16.0;
+  double zstab = 0.0;  // stability interval of the z-step polynomial
+  std::vector<Scen> sc;
+  std::vector<LineSet> sets;
+  int nbh_total = 0;
+  std::vector<PipeCol> pcol;
+  std::vector<int> bh_col_in0, bh_n_in, bh_col_an0, bh_n_an, bh_scen;
+  // c-independent device data
+  std::vector<void*> allocs;
+  ScenDev* d_sc = nullptr;
+  int64_t *d_cell_prefix = nullptr, *d_col_prefix = nullptr;
+  LineTile *d_xtiles = nullptr, *d_ytiles = nullptr;
+  uint16_t* d_cls = nullptr;
+  int32_t* d_zone = nullptr;
+  double* d_zcoef = nullptr;
+  int32_t* d_setid = nullptr;
+  PipeCol* d_pcol = nullptr;
+  double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
+  int nx_max = 0, ny_max = 0, nz_max = 0, countz_max = 0;
+  int64_t cells_total = 0, cols_total = 0, xtiles_total = 0, ytiles_total = 0;
+  std::map<double, CTab> ctabs;
+  bool host_inlet = false;
+  // captured step graph
+  hipGraphExec_t graph_exec = nullptr;
+  void* graph_phi = nullptr;
+  double graph_dt = 0;
+
+  template <class T>
+  T* up(const std::vector<T>& h) {
+    T* d = dev_upload(h);
+    allocs.push_back(d);
+    return d;
+  }
+  ~gw_ctx() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    for (auto& kv : ctabs) {
+      (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
+      (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
+    }
+    for (void* p : allocs) (void)hipFree(p);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+  }
+  DevView view(const CTab* t) const {
+    DevView v{};
+    v.sc = d_sc; v.nscen = (int)sc.size();
+    v.cell_prefix = d_cell_prefix; v.col_prefix = d_col_prefix;
+    v.xtiles = d_xtiles; v.ytiles = d_ytiles;
+    v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
+    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means; }
+    v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size();
+    v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
+    v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
+    v.cells_total = cells_total; v.cols_total = cols_total;
+    v.xtiles_total = xtiles_total; v.ytiles_total = ytiles_total;
+    return v;
+  }
+};
+
+namespace {
+
+constexpr int kTileL = 64;
+
+// Classification, zones and z coefficients of one scenario.
+void classify(Scen& s) {
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  const size_t nb = s.bh.size();
+  // boreholes whose skip radius can contain the column
+  std::vector<std::vector<int>> near((size_t)nx * ny);
+  for (size_t b = 0; b < nb; ++b) {
+    const HostBh& bh = s.bh[b];
+    const double R = std::sqrt(std::max(bh.r_backfill_sq, bh.r_and_t_outer_sq)) * 1.000001 + 1e-12;
+    const int i0 = (int)(std::lower_bound(s.gx.begin(), s.gx.end(), bh.xc - R) - s.gx.begin());
+    const int i1 = (int)(std::upper_bound(s.gx.begin(), s.gx.end(), bh.xc + R) - s.gx.begin());
+    const int j0 = (int)(std::lower_bound(s.gy.begin(), s.gy.end(), bh.yc - R) - s.gy.begin());
+    const int j1 = (int)(std::upper_bound(s.gy.begin(), s.gy.end(), bh.yc + R) - s.gy.begin());
+    for (int i = std::max(i0 - 1, 0); i < std::min(i1 + 1, nx); ++i)
+      for (int j = std::max(j0 - 1, 0); j < std::min(j1 + 1, ny); ++j) {
+        const double dx = s.gx[i] - bh.xc, dy = s.gy[j] - bh.yc;
+        const double r_sq = dx * dx + dy * dy;
+        if (!(r_sq > bh.r_backfill_sq) || !(r_sq > bh.r_and_t_outer_sq))
+          near[(size_t)j + (size_t)ny * i].push_back((int)b);
+      }
+  }
+  // class 0: plain rock
+  std::unordered_map<std::string, int> cmap;
+  s.kprof.clear(); s.rprof.clear();
+  std::vector<double> kp(nz), rp(nz);
+  const std::vector<int> none;
+  auto add_profile = [&](void) -> int {
+    std::string key(reinterpret_cast<const char*>(kp.data()), nz * sizeof(double));
+    key.append(reinterpret_cast<const char*>(rp.data()), nz * sizeof(double));
+    auto it = cmap.find(key);
+    if (it != cmap.end()) return it->second;
+    const int id = (int)cmap.size();
+    cmap.emplace(std::move(key), id);
+    s.kprof.insert(s.kprof.end(), kp.begin(), kp.end());
+    s.rprof.insert(s.rprof.end(), rp.begin(), rp.end());
+    return id;
+  };
+  for (int k = 0; k < nz; ++k) point_material(0, 0, s.gz[k], s.bh, none, s.mat, kp[k], rp[k]);
+  add_profile();
+  s.cls.assign((size_t)nx * ny, 0);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const auto& nr = near[(size_t)j + (size_t)ny * i];
+      if (nr.empty()) continue;
+      for (int k = 0; k < nz; ++k) point_material(s.gx[i], s.gy[j], s.gz[k], s.bh, nr, s.mat, kp[k], rp[k]);
+      const int id = add_profile();
+      if (id > 65535) throw std::runtime_error("more than 65536 material column classes");
+      s.cls[(size_t)j + (size_t)ny * i] = (uint16_t)id;
+    }
+  s.ncls = (int)cmap.size();
+  // zones
+  std::unordered_map<std::string, int> zmap;
+  s.zone.assign(nz, 0);
+  s.zone_rep.clear();
+  std::vector<double> key(2 * s.ncls);
+  for (int k = 0; k < nz; ++k) {
+    for (int c = 0; c < s.ncls; ++c) {
+      key[2 * c] = s.kprof[(size_t)c * nz + k];
+      key[2 * c + 1] = s.rprof[(size_t)c * nz + k];
+    }
+    std::string kk(reinterpret_cast<const char*>(key.data()), key.size() * sizeof(double));
+    auto it = zmap.find(kk);
+    if (it == zmap.end()) {
+      const int id = (int)zmap.size();
+      zmap.emplace(std::move(kk), id);
+      s.zone_rep.push_back(k);
+      s.zone[k] = id;
+    } else {
+      s.zone[k] = it->second;
+    }
+  }
+  s.nzone = (int)zmap.size();
+  // z coefficients per (class, level)
+  s.zcoef.assign((size_t)s.ncls * nz * 2, 0.0);
+  std::vector<double> ea(nz), eb(nz);
+  for (int c = 0; c < s.ncls; ++c) {
+    line_coefficients(s.gz.data(), nz, &s.kprof[(size_t)c * nz], &s.rprof[(size_t)c * nz], ea.data(), eb.data());
+    for (int k = 0; k < nz; ++k) {
+      s.zcoef[2 * ((size_t)c * nz + k)] = ea[k];
+      s.zcoef[2 * ((size_t)c * nz + k) + 1] = eb[k];
+      s.zrho = std::max(s.zrho, 2.0 * (ea[k] + eb[k]));
+    }
+  }
+}
+
+// x- and y-line coefficient sets (deduplicated across lines, zones and scenarios).
+void build_line_sets(gw_ctx& ctx, Scen& s, std::unordered_map<std::string, int>& smap) {
+  auto add_set = [&](int N, const std::vector<double>& ea, const std::vector<double>& eb) -> int {
+    std::string key(reinterpret_cast<const char*>(&N), sizeof N);
+    key.append(reinterpret_cast<const char*>(ea.data()), N * sizeof(double));
+    key.append(reinterpret_cast<const char*>(eb.data()), N * sizeof(double));
+    auto it = smap.find(key);
+    if (it != smap.end()) return it->second;
+    const int id = (int)ctx.sets.size();
+    smap.emplace(std::move(key), id);
+    ctx.sets.push_back(LineSet{N, ea, eb});
+    return id;
+  };
+  const int nx = s.nx, ny = s.ny, nz = s.nz;
+  s.xset.assign((size_t)ny * s.nzone, 0);
+  s.yset.assign((size_t)nx * s.nzone, 0);
+  std::vector<double> kc(std::max(nx, ny)), rc(std::max(nx, ny)), ea(std::max(nx, ny)), eb(std::max(nx, ny));
+  for (int z = 0; z < s.nzone; ++z) {
+    const int kr = s.zone_rep[z];
+    for (int j = 0; j < ny; ++j) {  // x lines
+      for (int i = 0; i < nx; ++i) {
+        const int c = s.cls[(size_t)j + (size_t)ny * i];
+        kc[i] = s.kprof[(size_t)c * nz + kr];
+        rc[i] = s.rprof[(size_t)c * nz + kr];
+      }
+      line_coefficients(s.gx.data(), nx, kc.data(), rc.data(), ea.data(), eb.data());
+      std::vector<double> a(ea.begin(), ea.begin() + nx), b(eb.begin(), eb.begin() + nx);
+      s.xset[(size_t)j * s.nzone + z] = add_set(nx, a, b);
+    }
+    for (int i = 0; i < nx; ++i) {  // y lines
+      for (int j = 0; j < ny; ++j) {
+        const int c = s.cls[(size_t)j + (size_t)ny * i];
+        kc[j] = s.kprof[(size_t)c * nz + kr];
+        rc[j] = s.rprof[(size_t)c * nz + kr];
+      }
+      line_coefficients(s.gy.data(), ny, kc.data(), rc.data(), ea.data(), eb.data());
+      std::vector<double> a(ea.begin(), ea.begin() + ny), b(eb.begin(), eb.begin() + ny);
+      s.yset[(size_t)i * s.nzone + z] = add_set(ny, a, b);
+    }
+  }
+}
+
+// Line tiles of one scenario and direction: 64-wide k ranges of every line,
+// split where the coefficient set changes (material zone boundaries in z).
+void line_tiles(const Scen& s, int si, int dir, std::vector<LineTile>& out) {
+  const int nlines = dir == 0 ? s.ny : s.nx;
+  const std::vector<int32_t>& ids = dir == 0 ? s.xset : s.yset;
+  for (int line = 0; line < nlines; ++line)
+    for (int kc = 0; kc < s.nz; kc += kTileL) {
+      const int kend = std::min(s.nz, kc + kTileL);
+      int k = kc;
+      while (k < kend) {
+        const int sid = ids[(size_t)line * s.nzone + s.zone[k]];
+        int k2 = k + 1;
+        while (k2 < kend && ids[(size_t)line * s.nzone + s.zone[k2]] == sid) ++k2;
+        out.push_back(LineTile{si, line, k, k2 - k, sid, 0, 0, 0});
+        k = k2;
+      }
+    }
+}
+
+// create_advection_index_lists  (src/cache.jl:9-44)
+void index_lists(Scen& s) {
+  s.idx_inner.clear(); s.idx_outer.clear();
+  for (size_t b = 0; b < s.bh.size(); ++b) {
+    const HostBh& bh = s.bh[b];
+    for (int i = 0; i < s.nx; ++i)
+      for (int j = 0; j < s.ny; ++j) {
+        const double dx = s.gx[i] - bh.xc, dy = s.gy[j] - bh.yc;
+        const double r_sq = dx * dx + dy * dy;
+        const double rti = bh.r_inner + bh.t_inner;
+        if (r_sq < bh.r_inner * bh.r_inner) s.idx_inner.push_back({i, j, (int)b});
+        else if (rti * rti <= r_sq && r_sq < bh.r_outer * bh.r_outer) s.idx_outer.push_back({i, j, (int)b});
+      }
+  }
+  double hmax = -1e300;
+  for (auto& b : s.bh) hmax = std::max(hmax, b.h);
+  s.countz = 0;
+  for (double z : s.gz) if (z <= hmax) ++s.countz;
+  double dzmin = 1e300;
+  for (int k = 0; k + 1 < s.nz; ++k) dzmin = std::min(dzmin, s.gz[k + 1] - s.gz[k]);
+  s.eigen = 8 * s.mat.dmax() / (dzmin * dzmin);  // src/solver.jl:474-488
+}
+
+// interpolation_helper + gpu_searchsortedfirst (src/solver.jl:362-389), 0-based.
+void interp(const std::vector<double>& g, double dep, int& kl, int& kr, double& a) {
+  int lo = 0, hi = (int)g.size() - 1;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (g[mid] < dep) lo = mid + 1; else hi = mid;
+  }
+  kr = lo;
+  kl = std::max(kr - 1, 0);
+  kr = std::min(kr, (int)g.size() - 1);
+  a = (dep - g[kl]) / (g[kr] - g[kl]);
+}
+
+// Tables that depend on the sub-step length c.
+int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
+  auto it = ctx.ctabs.find(c);
+  if (it != ctx.ctabs.end()) { out = &it->second; return GW_OK; }
+  CTab t;
+  t.c = c;
+  // LU factors of (I - cA) per line set
+  std::vector<int64_t> set_off(ctx.sets.size());
+  int64_t tot = 0;
+  for (size_t q = 0; q < ctx.sets.size(); ++q) { set_off[q] = tot; tot += SET_NARR * (int64_t)ctx.sets[q].N; }
+  std::vector<double> coef((size_t)tot);
+  for (size_t q = 0; q < ctx.sets.size(); ++q) {
+    const LineSet& L = ctx.sets[q];
+    const int N = L.N;
+    double* fw = &coef[set_off[q]];
+    double* bd = fw + N;
+    double* bu = fw + 2 * N;
+    double* ea = fw + 3 * N;
+    double* eb = fw + 4 * N;
+    double piv = 1.0 + c * (L.ea[0] + L.eb[0]);
+    fw[0] = 0.0;
+    for (int l = 1; l < N; ++l) {
+      const double lower = -c * L.eb[l];
+      const double upper_prev = -c * L.ea[l - 1];
+      const double w = lower / piv;
+      bd[l - 1] = 1.0 / piv;
+      bu[l - 1] = upper_prev / piv;
+      fw[l] = w;
+      piv = (1.0 + c * (L.ea[l] + L.eb[l])) - w * upper_prev;
+    }
+    bd[N - 1] = 1.0 / piv;
+    bu[N - 1] = 0.0;
+    for (int l = 0; l < N; ++l) { ea[l] = L.ea[l]; eb[l] = L.eb[l]; }
+  }
+  // departure tables
+  std::vector<Dep> dep;
+  std::vector<BhDev> hb(ctx.nbh_total);
+  int gb = 0, mean_off = 0;
+  for (size_t si = 0; si < ctx.sc.size(); ++si) {
+    const Scen& s = ctx.sc[si];
+    for (size_t b = 0; b < s.bh.size(); ++b, ++gb) {
+      const HostBh& bh = s.bh[b];
+      BhDev& B = hb[gb];
+      std::memset(&B, 0, sizeof B);
+      B.scen = (int)si;
+      B.col_in0 = ctx.bh_col_in0[gb]; B.n_in = ctx.bh_n_in[gb];
+      B.col_an0 = ctx.bh_col_an0[gb]; B.n_an = ctx.bh_n_an[gb];
+      B.inlet_kind = s.inlet.kind;
+      B.inlet_value = s.inlet.value;
+      if (B.n_in == 0)
+        return fail(GW_EINVAL, "borehole " + std::to_string(b + 1) + " of scenario " + std::to_string(si) +
+                                   " has no inner-pipe grid columns: its outlet mean would be 0/0 (src/solver.jl:257)");
+      const double zmax = bh.v_inner * c;
+      int nout = 0;
+      for (int k = 0; k < s.nz; ++k) if (s.gz[k] <= zmax) ++nout;
+      // gz is sorted, so the outlet levels are k < nout
+      B.n_outlev = nout;
+      if (nout == 0) return fail(GW_EINVAL, "no outlet levels (gridz[1] > v_inner*dt)");
+      B.dep_off = (int)dep.size();
+      int klo = 1 << 30, khi = -1;
+      for (int k = 0; k < s.countz; ++k) {
+        const double z = s.gz[k];
+        Dep di{}, da{};
+        if (z > bh.h) {
+          di.type = DEP_KEEP; da.type = DEP_KEEP;
+        } else {
+          const double zd = std::fma(bh.v_inner, c, z);  // src/solver.jl:304
+          if (zd > bh.h) {
+            const double dt1 = (bh.h - z) / bh.v_inner;
+            const double dt2 = c - dt1;
+            const double zd2 = std::fma(-bh.v_outer, dt2, bh.h);  // :310
+            interp(s.gz, zd2, di.kl, di.kr, di.a);
+            di.type = DEP_TURN;
+            klo = std::min(klo, di.kl);
+            khi = std::max(khi, di.kr);
+          } else {
+            interp(s.gz, zd, di.kl, di.kr, di.a);
+            di.type = DEP_OWN;
+          }
+          if (di.kl == di.kr || !std::isfinite(di.a))
+            return fail(GW_EINVAL, "degenerate inner-pipe departure point (zero flow rate or a turnaround "
+                                   "departure above the surface): the reference yields NaN (src/solver.jl:310,373)");
+          const double za = std::fma(-bh.v_outer, c, z);  // :348
+          if (za <= 0.0) {
+            da.type = DEP_INLET;
+          } else {
+            interp(s.gz, za, da.kl, da.kr, da.a);
+            da.type = DEP_OWN;
+            if (da.kl == da.kr || !std::isfinite(da.a))
+              return fail(GW_EINVAL, "degenerate annulus departure point (src/solver.jl:352)");
+          }
+        }
+        dep.push_back(di);
+        dep.push_back(da);
+      }
+      if (khi >= 0) {
+        if (B.n_an == 0)
+          return fail(GW_EINVAL, "borehole " + std::to_string(b + 1) +
+                                     " has no annulus grid columns: the turnaround mean would be 0/0 (src/solver.jl:334)");
+        B.klo = klo;
+        B.nlev = khi - klo + 1;
+      } else {
+        B.klo = 0;
+        B.nlev = 0;
+      }
+      B.mean_off = mean_off;
+      mean_off += B.nlev;
+    }
+  }
+  try {
+    t.d_set_off = dev_upload(set_off);
+    t.d_coef = dev_upload(coef);
+    t.d_bh = dev_upload(hb);
+    t.d_dep = dev_upload(dep);
+    std::vector<double> m(std::max(mean_off, 1), 0.0);
+    t.d_means = dev_upload(m);
+    t.nmeans = mean_off;
+  } catch (HipFail& e) {
+    return fail(GW_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e.e));
+  }
+  t.h_bh = hb;
+  auto res = ctx.ctabs.emplace(c, t);
+  out = &res.first->second;
+  return GW_OK;
+}
+
+int hip_status(hipError_t e, const char* where) {
+  if (e == hipSuccess) return GW_OK;
+  return fail(GW_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Largest x with |R(-y)| <= 1 on [0, x] for R(z) = 1 + z + z^2/2 + c3 z^3.
+double stability_bound(double c3) {
+  double last = 0.0;
+  for (int q = 1; q <= 200000; ++q) {
+    const double y = -q * 1e-4;
+    const double r = 1 + y + 0.5 * y * y + c3 * y * y * y;
+    if (std::fabs(r) > 1 + 1e-12) break;
+    last = -y;
+  }
+  return last;
+}
+
+// One advection sub-step (advection!, src/solver.jl:250-264) on phi.
+int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau) {
+  DevView v = ctx.view(t);
+  int rc = hip_status(launch_pipe_reduce(v, ctx.prec, phi, ctx.stream), "pipe_reduce");
+  if (rc) return rc;
+  if (ctx.host_inlet) {
+    std::vector<double> tout(ctx.nbh_total), inl(ctx.nbh_total);
+    rc = hip_status(hipMemcpyAsync(tout.data(), ctx.d_tout, sizeof(double) * ctx.nbh_total,
+                                   hipMemcpyDeviceToHost, ctx.stream), "outlet copy");
+    if (rc) return rc;
+    rc = hip_status(hipMemcpyAsync(inl.data(), ctx.d_inlet, sizeof(double) * ctx.nbh_total,
+                                   hipMemcpyDeviceToHost, ctx.stream), "inlet copy");
+    if (rc) return rc;
+    rc = hip_status(hipStreamSynchronize(ctx.stream), "sync");
+    if (rc) return rc;
+    int gb = 0;
+    for (auto& s : ctx.sc) {
+      const int nb = (int)s.bh.size();
+      if (s.inlet.kind == GW_INLET_HOST)
+        for (int b = 0; b < nb; ++b) inl[gb + b] = s.inlet.fn(b + 1, &tout[gb], nb, tau, s.inlet.user);
+      gb += nb;
+    }
+    rc = hip_status(hipMemcpyAsync(ctx.d_inlet, inl.data(), sizeof(double) * ctx.nbh_total,
+                                   hipMemcpyHostToDevice, ctx.stream), "inlet upload");
+    if (rc) return rc;
+    rc = hip_status(hipStreamSynchronize(ctx.stream), "sync");
+    if (rc) return rc;
+  }
+  return hip_status(launch_pipe_advect(v, ctx.prec, phi, ctx.stream), "pipe_advect");
+}
+
+int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
+  CTab* tab = nullptr;
+  const double c = dt / 2;
+  int rc = build_ctab(ctx, c, tab);
+  if (rc) return rc;
+  DevView v = ctx.view(tab);
+  // explicit y  (src/solver.jl:444)
+  rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, phi, phi, c, ctx.stream), "explicit y");
+  if (rc) return rc;
+  rc = sub_step_advect(ctx, tab, phi, t);  // :447
+  if (rc) return rc;
+  // implicit x then explicit x  (:450, :455)
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, phi, phi, c, ctx.stream), "implicit+explicit x");
+  if (rc) return rc;
+  rc = sub_step_advect(ctx, tab, phi, t + dt / 2);  // :458
+  if (rc) return rc;
+  // implicit y  (:461)
+  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, phi, phi, c, ctx.stream), "implicit y");
+}
+
+// The z-step applies R(dt A_z) = 1 + z + z^2/2 + c3 z^3 at every ROCK2 degree:
+// ROCK2's longer polynomials agree with it through z^2/2 and differ by
+// table coefficients times (dt rho(A_z))^3.., negligible unless dt rho(A_z)
+// is O(1), which the stability check below excludes.
+int check_zstep(gw_ctx& ctx, double dt) {
+  for (auto& s : ctx.sc)
+    if (dt * s.zrho > ctx.zstab)
+      return fail(GW_EINVAL, "dt * rho(A_z) = " + std::to_string(dt * s.zrho) +
+                                 " exceeds the stability interval of the z-step polynomial");
+  return GW_OK;
+}
+
+int zstep(gw_ctx& ctx, void* phi, double dt) {
+  const int rc = check_zstep(ctx, dt);
+  if (rc) return rc;
+  return hip_status(launch_zstep(ctx.view(nullptr), ctx.prec, phi, dt, ctx.c3, ctx.stream), "zstep");
+}
+
+int create_impl(const gw_problem* p, gw_ctx** out) {
+  if (!p || !out) return fail(GW_EINVAL, "null argument");
+  if (p->n_scen <= 0 || !p->scen) return fail(GW_EINVAL, "no scenarios");
+  if (p->precision != GW_F64 && p->precision != GW_F32_STATE) return fail(GW_EINVAL, "unknown precision");
+  std::unique_ptr<gw_ctx> ctx(new gw_ctx());
+  ctx->prec = p->precision;
+  ctx->c3 = p->rock2_c3 > 0 ? p->rock2_c3 : 1.0 / 16.0;
+  ctx->zstab = stability_bound(ctx->c3);
+  std::unordered_map<std::string, int> smap;
+  int64_t off = 0;
+  for (int si = 0; si < p->n_scen; ++si) {
+    const gw_scenario& q = p->scen[si];
+    if (q.nx < 3 || q.ny < 3 || q.nz < 3) return fail(GW_EINVAL, "every grid needs at least 3 points");
+    if (!q.gridx || !q.gridy || !q.gridz) return fail(GW_EINVAL, "null grid");
+    if (q.n_bh <= 0 || !q.boreholes) return fail(GW_EINVAL, "no boreholes");
+    if (q.nz > 4096) return fail(GW_EINVAL, "nz > 4096 is not supported by the z-step kernel");
+    if (q.nx > 640 || q.ny > 640) return fail(GW_EINVAL, "nx or ny > 640 is not supported by the line kernels");
+    Scen s;
+    s.nx = q.nx; s.ny = q.ny; s.nz = q.nz;
+    s.gx.assign(q.gridx, q.gridx + q.nx);
+    s.gy.assign(q.gridy, q.gridy + q.ny);
+    s.gz.assign(q.gridz, q.gridz + q.nz);
+    for (int a = 0; a + 1 < q.nx; ++a) if (!(s.gx[a] < s.gx[a + 1])) return fail(GW_EINVAL, "gridx not increasing");
+    for (int a = 0; a + 1 < q.ny; ++a) if (!(s.gy[a] < s.gy[a + 1])) return fail(GW_EINVAL, "gridy not increasing");
+    for (int a = 0; a + 1 < q.nz; ++a) if (!(s.gz[a] < s.gz[a + 1])) return fail(GW_EINVAL, "gridz not increasing");
+    for (int b = 0; b < q.n_bh; ++b) s.bh.push_back(make_bh(q.boreholes[b]));
+    const gw_materials& m = q.materials;
+    if (m.n_layers > 0) {
+      if (!m.k_rock_layers || !m.rho_c_rock_layers || !m.layer_depths) return fail(GW_EINVAL, "null layer arrays");
+      s.mat.kl.assign(m.k_rock_layers, m.k_rock_layers + m.n_layers);
+      s.mat.rl.assign(m.rho_c_rock_layers, m.rho_c_rock_layers + m.n_layers);
+      s.mat.depth.assign(m.layer_depths, m.layer_depths + m.n_layers);
+    }
+    s.mat.k_rock = m.k_rock; s.mat.rc_rock = m.rho_c_rock;
+    s.mat.k_water = m.k_water; s.mat.rc_water = m.rho_c_water;
+    s.mat.k_steel = m.k_steel; s.mat.rc_steel = m.rho_c_steel;
+    s.mat.k_ins = m.k_insulating; s.mat.rc_ins = m.rho_c_insulating;
+    s.mat.k_bf = m.k_backfill; s.mat.rc_bf = m.rho_c_backfill;
+    s.inlet = q.inlet;
+    if (q.inlet.kind < 0 || q.inlet.kind > 3) return fail(GW_EINVAL, "unknown inlet kind");
+    if (q.inlet.kind == GW_INLET_PER_WELL) {
+      if (!q.inlet.per_well) return fail(GW_EINVAL, "per-well inlet table is null");
+      s.per_well.assign(q.inlet.per_well, q.inlet.per_well + q.n_bh);
+    }
+    if (q.inlet.kind == GW_INLET_HOST) {
+      if (!q.inlet.fn) return fail(GW_EINVAL, "host inlet callback is null");
+      ctx->host_inlet = true;
+    }
+    try {
+      classify(s);
+    } catch (std::exception& e) {
+      return fail(GW_EINVAL, e.what());
+    }
+    index_lists(s);
+    build_line_sets(*ctx, s, smap);
+    s.off = off;
+    off += (int64_t)s.nx * s.ny * s.nz;
+    ctx->sc.push_back(std::move(s));
+  }
+  // pipe columns grouped per borehole, inner first
+  int gb = 0;
+  for (auto& s : ctx->sc) {
+    for (size_t b = 0; b < s.bh.size(); ++b, ++gb) {
+      ctx->bh_scen.push_back((int)(&s - &ctx->sc[0]));
+      ctx->bh_col_in0.push_back((int)ctx->pcol.size());
+      int n_in = 0;
+      for (auto& e : s.idx_inner) if (e[2] == (int)b) { ctx->pcol.push_back(PipeCol{e[0], e[1], gb, 1}); ++n_in; }
+      ctx->bh_n_in.push_back(n_in);
+      ctx->bh_col_an0.push_back((int)ctx->pcol.size());
+      int n_an = 0;
+      for (auto& e : s.idx_outer) if (e[2] == (int)b) { ctx->pcol.push_back(PipeCol{e[0], e[1], gb, 0}); ++n_an; }
+      ctx->bh_n_an.push_back(n_an);
+    }
+  }
+  ctx->nbh_total = gb;
+  // device upload
+  try {
+    if (p->stream) {
+      ctx->stream = (hipStream_t)p->stream;
+    } else {
+      hip_check(hipStreamCreate(&ctx->stream));
+      ctx->own_stream = true;
+    }
+    std::vector<ScenDev> sd;
+    std::vector<int64_t> cellp{0}, colp{0};
+    std::vector<LineTile> xt, yt;
+    std::vector<uint16_t> cls;
+    std::vector<int32_t> zone, setid;
+    std::vector<double> zcoef, inlet_tab;
+    int gbh = 0;
+    for (auto& s : ctx->sc) {
+      ScenDev d{};
+      d.nx = s.nx; d.ny = s.ny; d.nz = s.nz; d.countz = s.countz;
+      d.nzone = s.nzone; d.ncls = s.ncls; d.nbh = (int)s.bh.size(); d.bh0 = gbh;
+      d.off = s.off;
+      d.o_cls = (int64_t)cls.size(); cls.insert(cls.end(), s.cls.begin(), s.cls.end());
+      d.o_zone = (int64_t)zone.size(); zone.insert(zone.end(), s.zone.begin(), s.zone.end());
+      d.o_zcoef = (int64_t)zcoef.size() / 2; zcoef.insert(zcoef.end(), s.zcoef.begin(), s.zcoef.end());
+      d.o_xset = (int64_t)setid.size(); setid.insert(setid.end(), s.xset.begin(), s.xset.end());
+      d.o_yset = (int64_t)setid.size(); setid.insert(setid.end(), s.yset.begin(), s.yset.end());
+      sd.push_back(d);
+      const int64_t ncell = (int64_t)s.nx * s.ny * s.nz;
+      cellp.push_back(cellp.back() + ncell);
+      colp.push_back(colp.back() + (int64_t)s.nx * s.ny);
+      line_tiles(s, (int)sd.size() - 1, 0, xt);
+      line_tiles(s, (int)sd.size() - 1, 1, yt);
+      for (size_t b = 0; b < s.bh.size(); ++b)
+        inlet_tab.push_back(s.inlet.kind == GW_INLET_PER_WELL ? s.per_well[b] : 0.0);
+      gbh += (int)s.bh.size();
+      ctx->nx_max = std::max(ctx->nx_max, s.nx);
+      ctx->ny_max = std::max(ctx->ny_max, s.ny);
+      ctx->nz_max = std::max(ctx->nz_max, s.nz);
+      ctx->countz_max = std::max(ctx->countz_max, s.countz);
+    }
+    ctx->cells_total = cellp.back();
+    ctx->cols_total = colp.back();
+    ctx->xtiles_total = (int64_t)xt.size();
+    ctx->ytiles_total = (int64_t)yt.size();
+    ctx->d_sc = ctx->up(sd);
+    ctx->d_cell_prefix = ctx->up(cellp);
+    ctx->d_col_prefix = ctx->up(colp);
+    ctx->d_xtiles = ctx->up(xt);
+    ctx->d_ytiles = ctx->up(yt);
+    ctx->d_cls = ctx->up(cls);
+    ctx->d_zone = ctx->up(zone);
+    ctx->d_zcoef = ctx->up(zcoef);
+    ctx->d_setid = ctx->up(setid);
+    ctx->d_pcol = ctx->up(ctx->pcol);
+    std::vector<double> zeros(std::max(ctx->nbh_total, 1), 0.0);
+    ctx->d_tout = ctx->up(zeros);
+    ctx->d_inlet = ctx->up(zeros);
+    ctx->d_inlet_tab = ctx->up(inlet_tab);
+  } catch (HipFail& e) {
+    return fail(GW_EHIP, std::string("device set-up failed: ") + hipGetErrorString(e.e));
+  }
+  *out = ctx.release();
+  return GW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gw_last_error(void) { return g_err.c_str(); }
+
+int gw_create(const gw_problem* p, gw_ctx** out) {
+  try {
+    return create_impl(p, out);
+  } catch (std::exception& e) {
+    return fail(GW_EINVAL, e.what());
+  } catch (...) {
+    return fail(GW_EHIP, "unexpected failure in gw_create");
+  }
+}
+
+int gw_destroy(gw_ctx* ctx) {
+  if (!ctx) return GW_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return GW_OK;
+}
+
+int gw_rhs_z(gw_ctx* ctx, void* dphi, const void* phi, double /*t*/) {
+  if (!ctx || !dphi || !phi) return fail(GW_EINVAL, "null argument");
+  return hip_status(launch_zrhs(ctx->view(nullptr), ctx->prec, dphi, phi, ctx->stream), "rhs_z");
+}
+
+int gw_adi_adv(gw_ctx* ctx, void* phi, void* /*scratch*/, double t, double dt) {
+  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
+  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
+  return adi_adv(*ctx, phi, t, dt);
+}
+
+int gw_zstep(gw_ctx* ctx, void* phi, double dt) {
+  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
+  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
+  return zstep(*ctx, phi, dt);
+}
+
+int gw_step(gw_ctx* ctx, void* phi, double t, double dt) {
+  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
+  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
+  int rc = zstep(*ctx, phi, dt);
+  if (rc) return rc;
+  return adi_adv(*ctx, phi, t + dt, dt);
+}
+
+int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
+  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
+  if (!(dt > 0) || nsteps < 0) return fail(GW_EINVAL, "dt must be positive and nsteps >= 0");
+  if (nsteps == 0) return GW_OK;
+  if (ctx->host_inlet) {
+    for (int n = 0; n < nsteps; ++n) {
+      const int rc = gw_step(ctx, phi, t + n * dt, dt);
+      if (rc) return rc;
+    }
+    return GW_OK;
+  }
+  // Every launch of a step is independent of t for device-side inlets, so
+  // one captured graph is replayed.
+  if (!ctx->graph_exec || ctx->graph_phi != phi || ctx->graph_dt != dt) {
+    if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
+    CTab* tab = nullptr;
+    int rc = build_ctab(*ctx, dt / 2, tab);  // host work outside the capture
+    if (rc) return rc;
+    rc = check_zstep(*ctx, dt);
+    if (rc) return rc;
+    hipGraph_t g = nullptr;
+    rc = hip_status(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    const int rc1 = gw_step(ctx, phi, t, dt);
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    if (rc1) { if (g) (void)hipGraphDestroy(g); return rc1; }
+    rc = hip_status(e2, "end capture");
+    if (rc) return rc;
+    rc = hip_status(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    ctx->graph_phi = phi;
+    ctx->graph_dt = dt;
+  }
+  for (int n = 0; n < nsteps; ++n) {
+    const int rc = hip_status(hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch");
+    if (rc) return rc;
+  }
+  return GW_OK;
+}
+
+int gw_outlet(gw_ctx* ctx, double* T_out) {
+  if (!ctx || !T_out) return fail(GW_EINVAL, "null argument");
+  int rc = hip_status(hipMemcpyAsync(T_out, ctx->d_tout, sizeof(double) * ctx->nbh_total,
+                                     hipMemcpyDeviceToHost, ctx->stream), "outlet copy");
+  if (rc) return rc;
+  return hip_status(hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int gw_set_inlet(gw_ctx* ctx, const double* T_in) {
+  if (!ctx || !T_in) return fail(GW_EINVAL, "null argument");
+  int gb = 0;
+  for (auto& s : ctx->sc) {
+    if (s.inlet.kind == GW_INLET_PER_WELL)
+      for (size_t b = 0; b < s.bh.size(); ++b) s.per_well[b] = T_in[gb + b];
+    gb += (int)s.bh.size();
+  }
+  std::vector<double> tab(ctx->nbh_total, 0.0);
+  gb = 0;
+  for (auto& s : ctx->sc) {
+    for (size_t b = 0; b < s.bh.size(); ++b) tab[gb + b] = s.inlet.kind == GW_INLET_PER_WELL ? s.per_well[b] : 0.0;
+    gb += (int)s.bh.size();
+  }
+  int rc = hip_status(hipMemcpyAsync(ctx->d_inlet_tab, tab.data(), sizeof(double) * tab.size(),
+                                     hipMemcpyHostToDevice, ctx->stream), "inlet table");
+  if (rc) return rc;
+  return hip_status(hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int gw_explicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
+  if (!ctx || !out || !in || (dir != 0 && dir != 1)) return fail(GW_EINVAL, "bad argument");
+  CTab* tab = nullptr;
+  int rc = build_ctab(*ctx, c, tab);
+  if (rc) return rc;
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, out, in, c, ctx->stream), "explicit");
+}
+
+int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
+  if (!ctx || !out || !in || (dir != 0 && dir != 1)) return fail(GW_EINVAL, "bad argument");
+  CTab* tab = nullptr;
+  int rc = build_ctab(*ctx, c, tab);
+  if (rc) return rc;
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, out, in, c, ctx->stream), "implicit");
+}
+
+int gw_advect(gw_ctx* ctx, void* phi, double c, double t) {
+  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
+  CTab* tab = nullptr;
+  int rc = build_ctab(*ctx, c, tab);
+  if (rc) return rc;
+  return sub_step_advect(*ctx, tab, phi, t);
+}
+
+int64_t gw_state_elems(gw_ctx* ctx) { return ctx ? ctx->cells_total : -1; }
+
+int64_t gw_scenario_offset(gw_ctx* ctx, int32_t s) {
+  if (!ctx || s < 0 || s >= (int)ctx->sc.size()) return -1;
+  return ctx->sc[s].off;
+}
+
+int gw_counts(gw_ctx* ctx, int32_t s, int32_t* countz, int32_t* n_inner, int32_t* n_outer) {
+  if (!ctx || s < 0 || s >= (int)ctx->sc.size()) return fail(GW_EINVAL, "bad scenario");
+  if (countz) *countz = ctx->sc[s].countz;
+  if (n_inner) *n_inner = (int32_t)ctx->sc[s].idx_inner.size();
+  if (n_outer) *n_outer = (int32_t)ctx->sc[s].idx_outer.size();
+  return GW_OK;
+}
+
+int gw_index_lists(gw_ctx* ctx, int32_t s, int32_t* inner, int32_t* outer) {
+  if (!ctx || s < 0 || s >= (int)ctx->sc.size()) return fail(GW_EINVAL, "bad scenario");
+  const Scen& sc = ctx->sc[s];
+  if (inner)
+    for (size_t c = 0; c < sc.idx_inner.size(); ++c)
+      for (int q = 0; q < 3; ++q) inner[3 * c + q] = sc.idx_inner[c][q];
+  if (outer)
+    for (size_t c = 0; c < sc.idx_outer.size(); ++c)
+      for (int q = 0; q < 3; ++q) outer[3 * c + q] = sc.idx_outer[c][q];
+  return GW_OK;
+}
+
+double gw_eigen_estimate(gw_ctx* ctx, int32_t s) {
+  if (!ctx || s < 0 || s >= (int)ctx->sc.size()) return -1.0;
+  return ctx->sc[s].eigen;
+}
+
+int gw_sync(gw_ctx* ctx) {
+  if (!ctx) return fail(GW_EINVAL, "null argument");
+  return hip_status(hipStreamSynchronize(ctx->stream), "sync");
+}
+
+}  // extern "C"

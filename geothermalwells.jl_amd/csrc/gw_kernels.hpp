@@ -1,0 +1,49 @@
+// gw_kernels.hpp -- host-callable launchers of the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gw_layout.hpp"
+
+namespace gw {
+
+// Device view of one context's tables, passed by value to every launcher.
+struct DevView {
+  const ScenDev* sc;
+  int32_t nscen;
+  const int64_t* cell_prefix;   // [nscen+1] cells
+  const int64_t* col_prefix;    // [nscen+1] (i, j) columns
+  const LineTile* xtiles;       // x-line tiles
+  const LineTile* ytiles;       // y-line tiles
+  const uint16_t* cls;
+  const int32_t* zone;
+  const double* zcoef;          // pairs {ea, eb}
+  const int32_t* setid;
+  // dt-dependent
+  const int64_t* set_off;
+  const double* coef;
+  const BhDev* bh;
+  int32_t nbh;
+  const PipeCol* pcol;
+  int32_t npcol;
+  const Dep* dep;
+  double* means;
+  double* tout;                 // [nbh] outlet of the current sub-step
+  double* inlet;                // [nbh] inlet value of the current sub-step
+  const double* inlet_tab;      // [nbh] per-well table
+  int32_t nx_max, ny_max, nz_max, countz_max;
+  int64_t cells_total, cols_total, xtiles_total, ytiles_total;
+};
+
+enum LineMode { LINE_EXPLICIT = 0, LINE_IMPLICIT = 1, LINE_IMPLICIT_EXPLICIT = 2 };
+
+hipError_t launch_zrhs(const DevView& v, int prec, void* dphi, const void* phi, hipStream_t st);
+hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double c3, hipStream_t st);
+hipError_t launch_line(const DevView& v, int prec, int mode, int dir, void* out, const void* in,
+                       double c, hipStream_t st);
+// Outlet sums, annulus level means and inlet values of one sub-step.
+hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st);
+// Semi-Lagrangian update of every pipe column, in place.
+hipError_t launch_pipe_advect(const DevView& v, int prec, void* phi, hipStream_t st);
+
+}  // namespace gw

@@ -1,0 +1,156 @@
+/* geowells.h -- C ABI of libgeowells_hip, the gfx950 (MI355X) implementation of
+ * GeothermalWells.jl's per-timestep hot path.
+ *
+ * The Julia package keeps its API (create_cache, rhs_diffusion_z!, the
+ * ODEProblem/ROCK2 driver, get_simulation_callback); each entry point below
+ * replaces the reference function named in its comment (paths relative to the
+ * GeothermalWells.jl v0.1.3 source tree).  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - Return value: GW_OK (0) or a negative gw_status; gw_last_error() gives a
+ *    message for the calling thread.  Nothing aborts or throws across the ABI.
+ *  - State buffers are device pointers owned by the caller, valid only during
+ *    the call.  A field is the reference's column-major phi[k, j, i] of size
+ *    (Nz, Ny, Nx), element type double (GW_F64) or float (GW_F32_STATE).  A
+ *    batch packs its scenarios back to back (gw_scenario_offset).
+ *  - A context is bound to one HIP stream and is not re-entrant.  Calls are
+ *    stream-ordered and asynchronous except gw_outlet, gw_create and a
+ *    sub-step whose inlet is evaluated on the host (GW_INLET_HOST).
+ */
+#ifndef GEOWELLS_H
+#define GEOWELLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gw_status {
+  GW_OK = 0,
+  GW_EINVAL = -1,  /* invalid argument or a configuration the reference turns into NaN */
+  GW_EHIP = -2,    /* HIP runtime error */
+  GW_ENOMEM = -3,  /* device allocation failed */
+  GW_ESTATE = -4   /* call not valid in the context's current state */
+} gw_status;
+
+enum { GW_F64 = 0, GW_F32_STATE = 1 };
+
+/* Inlet models (src/physics.jl:283-318). */
+enum {
+  GW_INLET_CONSTANT = 0,       /* ConstantInlet(T):        T                       */
+  GW_INLET_HEAT_EXCHANGER = 1, /* HeatExchangerInlet(dT):  T_outlet[bh] - dT       */
+  GW_INLET_PER_WELL = 2,       /* CustomInlet((bh,_,_) -> T_in[bh]) as a table   */
+  GW_INLET_HOST = 3            /* CustomInlet(f): f evaluated on the host per sub-step */
+};
+
+/* Borehole{T}(xc, yc, h, r_inner, t_inner, r_outer, t_outer, r_backfill, mdot,
+ * insulation_depth) constructor arguments (src/physics.jl:48). */
+typedef struct gw_borehole {
+  double xc, yc, h, r_inner, t_inner, r_outer, t_outer, r_backfill, mdot, insulation_depth;
+} gw_borehole;
+
+/* HomogenousMaterialProperties (src/physics.jl:159-170) when n_layers == 0,
+ * else StratifiedMaterialProperties{n_layers} (src/physics.jl:81-94). */
+typedef struct gw_materials {
+  int32_t n_layers;
+  const double *k_rock_layers, *rho_c_rock_layers, *layer_depths;
+  double k_rock, rho_c_rock;
+  double k_water, rho_c_water, k_steel, rho_c_steel;
+  double k_insulating, rho_c_insulating, k_backfill, rho_c_backfill;
+} gw_materials;
+
+/* Host inlet callback: (1-based borehole index, outlet temperatures of this
+ * scenario's boreholes, their count, time) -> inlet temperature. */
+typedef double (*gw_inlet_fn)(int32_t bh, const double *T_outlet, int32_t n_bh, double t,
+                              void *user);
+
+typedef struct gw_inlet {
+  int32_t kind;            /* GW_INLET_* */
+  double value;            /* CONSTANT: T; HEAT_EXCHANGER: dT */
+  const double *per_well;  /* PER_WELL: n_bh values */
+  gw_inlet_fn fn;          /* HOST */
+  void *user;
+} gw_inlet;
+
+/* One scenario: the arguments of create_cache (src/cache.jl:55). */
+typedef struct gw_scenario {
+  int32_t nx, ny, nz;
+  const double *gridx, *gridy, *gridz;
+  int32_t n_bh;
+  const gw_borehole *boreholes;
+  gw_materials materials;
+  gw_inlet inlet;
+} gw_scenario;
+
+typedef struct gw_problem {
+  int32_t n_scen;
+  const gw_scenario *scen;
+  int32_t precision;   /* GW_F64 or GW_F32_STATE */
+  void *stream;        /* hipStream_t; NULL: the context creates its own */
+  double rock2_c3;     /* cubic coefficient of the minimum-degree ROCK2 polynomial; <= 0: default 1/16 */
+  int32_t flags;       /* reserved, 0 */
+} gw_problem;
+
+typedef struct gw_ctx gw_ctx;
+
+/* create_cache (src/cache.jl:55-116) and create_advection_index_lists
+ * (src/cache.jl:9-44): classification, index lists, coefficient tables and
+ * eigen_estimator_pre_calculation (src/solver.jl:474-488).  Synchronous. */
+int gw_create(const gw_problem *p, gw_ctx **out);
+int gw_destroy(gw_ctx *ctx);
+const char *gw_last_error(void);
+
+/* rhs_diffusion_z!(dphi, phi, cache, t) (src/solver.jl:23-31): dphi = A_z phi. */
+int gw_rhs_z(gw_ctx *ctx, void *dphi, const void *phi, double t);
+
+/* ADI_and_ADV_callback!(integrator) (src/solver.jl:432-464) with
+ * t = integrator.t (the step's end) and dt = t - integrator.tprev.  Runs in
+ * place on phi; `scratch` stands for integrator.uprev, which the reference
+ * overwrites: it may be NULL and is not written. */
+int gw_adi_adv(gw_ctx *ctx, void *phi, void *scratch, double t, double dt);
+
+/* The ROCK2 z-step of one step at its minimum degree, fused:
+ * phi = R(dt*A_z) phi with R(z) = 1 + z + z^2/2 + c3 z^3 (the polynomial the
+ * examples' ROCK2(eigen_est=eigen_estimator) applies, src/solver.jl:23-31,
+ * 495).  GW_EINVAL when dt*eigen_estimate leaves the minimum degree. */
+int gw_zstep(gw_ctx *ctx, void *phi, double dt);
+
+/* One full step from t to t+dt: gw_zstep then gw_adi_adv(t+dt). */
+int gw_step(gw_ctx *ctx, void *phi, double t, double dt);
+
+/* nsteps full steps from t; replays one captured hipGraph per step when the
+ * inlet models allow it (every kind but GW_INLET_HOST). */
+int gw_steps(gw_ctx *ctx, void *phi, double t, double dt, int32_t nsteps);
+
+/* cache.T_outlet after the last advection sub-step (src/solver.jl:253-257):
+ * all boreholes of all scenarios in order.  Synchronises the stream. */
+int gw_outlet(gw_ctx *ctx, double *T_outlet_host);
+
+/* Replace the per-well inlet table (GW_INLET_PER_WELL scenarios only),
+ * values for every borehole of the batch in order. */
+int gw_set_inlet(gw_ctx *ctx, const double *T_in);
+
+/* Single operators, for callers that sequence the step themselves:
+ * explicit (I + c A_dir) (diffusion_1D! with plus_I, src/solver.jl:34-58),
+ * implicit (I - c A_dir)^-1 (thomas_I_minus_A!, src/solver.jl:111-232),
+ * dir 0 = x, 1 = y; in == out is allowed.  gw_advect is advection!
+ * (src/solver.jl:250-264) of one sub-step of length c at time t, in place. */
+int gw_explicit(gw_ctx *ctx, void *out, const void *in, double c, int32_t dir);
+int gw_implicit(gw_ctx *ctx, void *out, const void *in, double c, int32_t dir);
+int gw_advect(gw_ctx *ctx, void *phi, double c, double t);
+
+/* Introspection (cache fields the reference exposes). */
+int64_t gw_state_elems(gw_ctx *ctx);
+int64_t gw_scenario_offset(gw_ctx *ctx, int32_t scen);
+int gw_counts(gw_ctx *ctx, int32_t scen, int32_t *countz, int32_t *n_inner, int32_t *n_outer);
+/* Idx_list_Inner / Idx_list_Outer as 0-based (i, j, bh) triples. */
+int gw_index_lists(gw_ctx *ctx, int32_t scen, int32_t *inner, int32_t *outer);
+double gw_eigen_estimate(gw_ctx *ctx, int32_t scen);
+int gw_sync(gw_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEOWELLS_H */

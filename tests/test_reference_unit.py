@@ -1,0 +1,166 @@
+"""The reference's unit known-answers (test/test_basic.jl) against the host-side
+mirror of the API, and the oracle's own copies of the same functions."""
+import numpy as np
+import pytest
+
+import helpers as H
+
+
+def _bh(gw, *a, **kw):
+    return gw.Borehole(*a, **kw)
+
+
+def test_borehole_construction(gw, O):
+    # test/test_basic.jl:8-29
+    bh = gw.Borehole(0, 0, 3500, 0.0381, 0.01, 0.0889, 0.01, 0.0989, 10, 1000)
+    assert bh.h == 3500.0 and bh.ṁ == 10.0 and bh.insulation_depth == 1000.0
+    assert bh.v_inner > 0 and bh.v_outer > 0
+    # velocities v = mdot / (A rho_water) with the constructor's own areas
+    A_in = np.pi * 0.0381 ** 2
+    assert bh.v_inner == pytest.approx(10 / (A_in * 998.2), rel=1e-15)
+    f = O.borehole_fields(bh.args)
+    assert np.array_equal(f, np.array([getattr(bh, n) for n in gw.Borehole.FIELDS], dtype=np.float64))
+
+
+def test_material_constructors(gw):
+    # test/test_basic.jl:33-51
+    m = gw.HomogenousMaterialProperties(2.88, 2.17e6, 0.6, 4.18e6, 44.5, 3.73e6, 0.26, 1.96e6, 1.0, 1.0)
+    assert m.k_rock == 2.88 and m.k_water == 0.6
+
+
+def _li_materials(gw):
+    return gw.StratifiedMaterialProperties((1.8, 2.6, 3.5, 5.3),
+                                           (1780 * 1379.0, 2030 * 1450.0, 1510 * 1300.0, 2600 * 878.0),
+                                           (500.0, 1000.0, 1500.0, 2000.0), 0.618, 4.166e6, 41.0, 7850 * 475.0,
+                                           0.4, 1.955e6, 1.5, 1.76e6)
+
+
+def test_stratified_layers(gw, O):
+    # test/test_basic.jl:53-93
+    from geothermalwells_jl_amd import physics as P
+    m = _li_materials(gw)
+    assert m.k_rock_layers[0] == 1.8 and m.k_rock_layers[3] == 5.3 and m.layer_depths[1] == 1000.0
+    k = P.get_rock_thermal_conductivity_by_depth
+    rc = P.get_rock_volumetric_heat_capacity_by_depth
+    for z, kv in [(100.0, 1.8), (500.0, 1.8), (750.0, 2.6), (1250.0, 3.5), (1750.0, 5.3), (3000.0, 5.3)]:
+        assert k(z, m) == kv
+    for z, rv in [(100.0, 1780 * 1379.0), (750.0, 2030 * 1450.0), (1250.0, 1510 * 1300.0),
+                  (1750.0, 2600 * 878.0), (3000.0, 2600 * 878.0)]:
+        assert rc(z, m) == rv
+    d_layers = [m.k_rock_layers[i] / m.rho_c_rock_layers[i] for i in range(4)]
+    expect = max(max(d_layers), m.k_water / m.rho_c_water, m.k_steel / m.rho_c_steel,
+                 m.k_insulating / m.rho_c_insulating, m.k_backfill / m.rho_c_backfill)
+    assert P.eigen_estimator_get_dmax(m) == pytest.approx(expect)
+    # the oracle's lookups agree on the same values
+    bh = gw.Borehole(0.0, 0.0, 2000.0, 0.0511, 0.0114, 0.0885, 0.00833, 0.115, 11.65, 0.0)
+    gz = np.linspace(0, 2500, 11)
+    w = dict(gridx=np.linspace(-1, 1, 5), gridy=np.linspace(-1, 1, 5), gridz=gz, materials=m, boreholes=(bh,),
+             inlet_model=gw.ConstantInlet(20.0))
+    orc = H.oracle_for(w)
+    from oracle import lib as olib
+    for z in (100.0, 500.0, 750.0, 1250.0, 1750.0, 3000.0):
+        assert olib().or_rock_k(orc.h, z) == k(z, m)
+        assert olib().or_rock_rc(orc.h, z) == rc(z, m)
+    assert olib().or_dmax(orc.h) == pytest.approx(expect)
+
+
+def test_grid_creation_float32(gw):
+    # test/test_basic.jl:95-113
+    bh = gw.Borehole(0.0, 0.0, 2000 - 1, 0.0511, 0.0114, 0.0885, 0.00833, 0.115, 11.65, 0.0, dtype=np.float32)
+    d = gw.compute_domain([bh], buffer_x=100, buffer_y=100, buffer_z=200 + 1)
+    assert d.xmin == -100 and d.xmax == 100 and d.zmax == 2200
+    gz = gw.create_uniform_gridz_with_borehole_depths(zmin=0, zmax=2200, dz=100, boreholes=(bh,))
+    assert gz.dtype == np.float32 and bh.h in gz and gz[0] == 0
+
+
+def test_adaptive_grid_single(gw, O):
+    # test/test_basic.jl:115-130
+    bh = gw.Borehole(0.0, 0.0, 1999.0, 0.0511, 0.0114, 0.0885, 0.00833, 0.115, 11.65, 0.0)
+    g = gw.create_adaptive_grid_1d(xmin=-100, xmax=100, dx_fine=0.0025, growth_factor=1.3, dx_max=10.0,
+                                   boreholes=(bh,), direction="x")
+    assert g[0] == -100 and g[-1] == 100 and np.diff(g).min() <= 0.0025
+    assert np.array_equal(g, O.adaptive_grid(-100.0, 100.0, 0.0025, 1.3, 10.0, [bh.args], True))
+
+
+def test_adaptive_grid_multiple(gw, O):
+    # test/test_basic.jl:132-175
+    mk = lambda x, y: gw.Borehole(x, y, 2000.0, 0.0511, 0.0114, 0.0885, 0.00833, 0.115, 11.65, 0.0)
+    b1, b2 = mk(-15.0, 0.0), mk(15.0, 0.0)
+    g = gw.create_adaptive_grid_1d(xmin=-100, xmax=100, dx_fine=0.0025, growth_factor=1.3, dx_max=10.0,
+                                   boreholes=(b1, b2), direction="x")
+    assert g[0] == -100.0 and g[-1] == 100.0
+    assert np.all(np.diff(g) > 0) and len(np.unique(g)) == len(g)
+    assert np.any(np.abs(g - b1.xc) < b1.r_backfill) and np.any(np.abs(g - b2.xc) < b2.r_backfill)
+    between = g[(g > b1.xc + b1.r_backfill) & (g < b2.xc - b2.r_backfill)]
+    assert len(between) > 0
+    assert np.array_equal(g, O.adaptive_grid(-100.0, 100.0, 0.0025, 1.3, 10.0, [b1.args, b2.args], True))
+    b3, b4 = mk(0.0, -15.0), mk(0.0, 15.0)
+    gy = gw.create_adaptive_grid_1d(xmin=-100, xmax=100, dx_fine=0.0025, growth_factor=1.3, dx_max=10.0,
+                                    boreholes=(b3, b4), direction="y")
+    assert gy[0] == -100.0 and gy[-1] == 100.0 and np.all(np.diff(gy) > 0)
+    assert np.array_equal(gy, O.adaptive_grid(-100.0, 100.0, 0.0025, 1.3, 10.0, [b3.args, b4.args], False))
+
+
+def test_uniform_gridz_multiple_depths(gw, O):
+    # test/test_basic.jl:177-194
+    mk = lambda x, h: gw.Borehole(x, 0.0, h, 0.0511, 0.0114, 0.0885, 0.00833, 0.115, 11.65, 0.0)
+    b1, b2 = mk(-15.0, 1500.0), mk(15.0, 2000.0)
+    gz = gw.create_uniform_gridz_with_borehole_depths(zmin=0, zmax=2500, dz=100, boreholes=(b1, b2))
+    assert gz[0] == 0.0 and 1500.0 in gz and 2000.0 in gz
+    assert np.all(np.diff(gz) > 0)
+    assert np.array_equal(gz, O.uniform_gridz(0.0, 2500.0, 100.0, [b1.args, b2.args]))
+    # decimal steps are lifted to exact rationals as Julia does (0:2.1:2100 has 1001 points)
+    bh = mk(0.0, 2000.0)
+    g = gw.create_uniform_gridz_with_borehole_depths(zmin=0.0, zmax=2100.0, dz=2.1, boreholes=(bh,))
+    assert len(g) == 1002 and g[3] == 6.3 and g[-1] == 2100.0
+    assert np.array_equal(g, O.uniform_gridz(0.0, 2100.0, 2.1, [bh.args]))
+
+
+def test_initial_condition(gw):
+    # test/test_basic.jl:196-213
+    gx = np.arange(0.0, 100.0 + 1, 10.0)
+    gz = np.arange(0.0, 3700.0 + 1, 100.0)
+    T0 = gw.initial_condition_thermal_gradient("cpu", np.float64, gx, gx, gz, T_surface=2.29, gradient=0.035)
+    assert tuple(T0.shape) == (len(gx), len(gx), len(gz))
+    assert T0[0, 0, 0].item() == pytest.approx(2.29)
+    assert T0[0, 0, -1].item() == pytest.approx(2.29 + 0.035 * 3700.0)
+
+
+def test_inlet_models(gw):
+    # test/test_basic.jl:215-236
+    assert gw.ConstantInlet(20.0)(1, [0.0], 0.0) == 20.0
+    Q = 200e3
+    mdot = 42 * 998.2 / 3600
+    dT = Q / (mdot * 4174.0)
+    assert gw.HeatExchangerInlet(dT)(1, [50.0], 0.0) == pytest.approx(50.0 - dT)
+    assert gw.CustomInlet(lambda _, __, t: 20.0 + 0.01 * t)(1, [0.0], 100.0) == pytest.approx(21.0)
+    assert gw.PerWellInlet([3.0, 4.0])(2, [0.0, 0.0], 0.0) == 4.0
+
+
+def test_array_domain(gw):
+    # test/test_basic.jl:238-257
+    XC = YC = [-30.0, 0.0, 30.0]
+    bhs = tuple(gw.Borehole(x, y, 2000.0, 0.0511, 0.0114, 0.0885, 0.00833, 0.115, 11.65, 0.0)
+                for x in XC for y in YC)
+    assert len(bhs) == 9
+    d = gw.compute_domain(bhs, buffer_x=100, buffer_y=100, buffer_z=200)
+    assert (d.xmin, d.xmax, d.ymin, d.ymax) == (-130, 130, -130, 130)
+
+
+def test_point_material_classification(gw, O):
+    """The host mirror's point-wise lookups equal the oracle's at every cell of
+    a grid that crosses every radial zone, the plug and the insulation depth."""
+    from geothermalwells_jl_amd import physics as P
+    w = H.small_case(nbh=2, zins=20.0)
+    orc = H.oracle_for(w)
+    km, rcm = orc.material_maps()
+    bhs, m = w["boreholes"], w["materials"]
+    for i in range(0, len(w["gridx"]), 3):
+        for j in range(0, len(w["gridy"]), 2):
+            for k in range(0, len(w["gridz"]), 5):
+                x, y, z = w["gridx"][i], w["gridy"][j], w["gridz"][k]
+                assert km[i, j, k] == P.get_thermal_conductivity(x, y, z, bhs, m)
+                assert rcm[i, j, k] == P.get_volumetric_heat_capacity(x, y, z, bhs, m)
+    # every material value occurs
+    assert set(np.unique(km)) == {2.5, 0.6, 0.4, 44.5, 1.5}
+    assert set(np.unique(rcm)) == {2.3e6, 4.18e6, 1.96e6, 3.73e6}
