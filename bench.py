@@ -1,0 +1,164 @@
+"""Benchmark: cell-steps/s of the per-timestep hot path on one MI355X.
+
+Workload: the 3x3 borehole array (401 x 401 x 1002 cells, FP64, dt 300 s,
+per-well inlets) -- the plain-FP64 GPU configuration of the metric.  A step is
+one full dt: ROCK2 z-step, explicit-y, advection, implicit-x + explicit-x,
+advection, implicit-y.  The state stays resident in HBM; each step is one
+captured hipGraph replay.
+
+Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits: this
+build measures one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import __graft_entry__ as entry  # noqa: E402
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+METRIC = "cell-steps/s per GPU, plain FP64 vs batched/FP32 at ≤1e-3 K outlet err; %HBM BW"
+OPS = ["zstep", "explicit_y", "advect_1", "implicit_explicit_x", "advect_2", "implicit_y"]
+
+
+def pipe_bytes(cache, w, s):
+    """Algorithmic pipe bytes per step: 2 sub-steps x (advection read+write of
+    every pipe cell, outlet read, turnaround annulus level means)."""
+    bhs = w["boreholes"]
+    inner, outer = cache.index_lists(0)
+    n_in, n_an = len(inner), len(outer)
+    gz = np.asarray(w["gridz"])
+    c = w["dt"] / 2
+    n_out = sum(int(np.sum(gz <= float(b.v_inner) * c)) for b in bhs) / len(bhs)
+    countz = cache.countz
+    return 2 * ((n_in + n_an) * countz * 2 * s + n_in * n_out * s + n_an * countz * s)
+
+
+def cpu_baseline(w, threads, steps):
+    """The oracle (a restatement of the reference CPU path that keeps its
+    per-cell structure) on the host cores, on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers as H
+    O = entry.load_oracle()
+    O.set_threads(threads)
+    orc = H.oracle_for(w)
+    u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+    _, _, sec = orc.run(u0, 0.0, w["dt"], steps, 1.0 / 16.0)
+    cells = len(w["gridx"]) * len(w["gridy"]) * len(w["gridz"])
+    return dict(value=cells * steps / sec, unit="cell-steps/s", cores=threads, kind="port",
+                sample=f"{steps} step(s) of the same 401x401x1002 workload, {sec:.1f} s wall, "
+                       f"OpenMP restatement in oracle/ (point-wise material lookups over all boreholes, "
+                       f"line-per-thread Thomas, quadratic turnaround loop as in the reference)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device-to-device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the hot path is single-device in this build (see DESIGN.md on "
+              "sharding); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    gw = entry.load_package()
+    from geothermalwells_jl_amd import workloads as W
+    w = W.c1()
+    cells = W.cells(w)
+    cache = gw.create_cache(precision="f64", **W.scenario_kwargs(w))
+    lib = cache._lib
+    u = gw.initial_condition_thermal_gradient("cuda", np.float64, w["gridx"], w["gridy"], w["gridz"],
+                                              **w["ic"]).reshape(-1).contiguous()
+    ptr = cache.check_state(u)
+    dt = w["dt"]
+    torch.cuda.synchronize()
+    gw._abi.check(lib.gw_steps(cache.ctx, ptr, 0.0, dt, args.warmup))
+    cache.sync()
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.ExternalStream(lib.gw_stream(cache.ctx))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_start = time.perf_counter()
+    e0.record(stream)
+    gw._abi.check(lib.gw_steps(cache.ctx, ptr, args.warmup * dt, dt, args.steps))
+    e1.record(stream)
+    cache.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ev_ms = e0.elapsed_time(e1)
+    ms_per_step = 1e3 * wall / args.steps
+    value = cells * args.steps / wall
+    T_out = cache.T_outlet
+    assert np.all(np.isfinite(T_out)), "non-finite outlet temperatures"
+    state_ok = bool(torch.isfinite(u).all().item())
+
+    # per-operator HIP-event timing on the context's stream
+    ms = (ctypes.c_double * 6)()
+    t0 = (args.warmup + args.steps) * dt
+    gw._abi.check(lib.gw_profile_steps(cache.ctx, ptr, t0, dt, args.profile_steps, ms))
+    per_op = {OPS[q]: ms[q] / args.profile_steps for q in range(6)}
+    s = 8
+    field_ops = ["zstep", "explicit_y", "implicit_explicit_x", "implicit_y"]
+    dom = max(field_ops, key=lambda k: per_op[k])
+    dom_bytes = cells * 2 * s  # one read + one write of the state per launch
+    achieved = dom_bytes / (per_op[dom] * 1e-3)
+    B_cs = 10 * s
+    Bp = pipe_bytes(cache, w, s)
+    steps_per_s = args.steps / wall
+    step_bw = value * B_cs + steps_per_s * Bp
+    fused_bw = value * 8 * s + steps_per_s * Bp
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded inputs of the 3x3-array configuration; linear geothermal initial field)",
+        "config": {"workload": "3x3 borehole array, 401x401x1002 cells, FP64, dt 300 s, per-well inlets",
+                   "cells": cells, "boreholes": len(w["boreholes"]), "dt_s": dt, "graph_per_step": True},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": per_op[dom]},
+        "step_roofline": {"B_cs_5sweep": B_cs, "pipe_bytes_per_step": Bp,
+                          "frac_5sweep": step_bw / HBM_PEAK, "frac_4sweep_as_run": fused_bw / HBM_PEAK},
+        "per_op_ms": per_op,
+        "event_ms_per_step": ev_ms / args.steps,
+        "outlet_T": [float(x) for x in T_out],
+        "state_finite": state_ok,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

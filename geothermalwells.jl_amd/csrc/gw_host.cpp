@@ -191,7 +191,10 @@ struct CTab {
   BhDev* d_bh = nullptr;
   Dep* d_dep = nullptr;
   double* d_means = nullptr;
+  RedChunk* d_rchunk = nullptr;
+  double* d_partial = nullptr;
   int nmeans = 0;
+  int nrchunk = 0;
 };
 
 template <class T>
@@ -247,6 +250,7 @@ struct gw_ctx {
     for (auto& kv : ctabs) {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
+      (void)hipFree(kv.second.d_rchunk); (void)hipFree(kv.second.d_partial);
     }
     for (void* p : allocs) (void)hipFree(p);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -257,7 +261,8 @@ struct gw_ctx {
     v.cell_prefix = d_cell_prefix; v.col_prefix = d_col_prefix;
     v.xtiles = d_xtiles; v.ytiles = d_ytiles;
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
-    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means; }
+    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
+             v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial; }
     v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size();
     v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
     v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
@@ -490,7 +495,8 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   // departure tables
   std::vector<Dep> dep;
   std::vector<BhDev> hb(ctx.nbh_total);
-  int gb = 0, mean_off = 0;
+  std::vector<RedChunk> chunks;
+  int gb = 0, mean_off = 0, poff = 0;
   for (size_t si = 0; si < ctx.sc.size(); ++si) {
     const Scen& s = ctx.sc[si];
     for (size_t b = 0; b < s.bh.size(); ++b, ++gb) {
@@ -560,6 +566,23 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       }
       B.mean_off = mean_off;
       mean_off += B.nlev;
+      // reduction chunks: 64 pipe columns each
+      constexpr int kChunk = 64;
+      B.red_in0 = (int)chunks.size();
+      for (int c0 = 0; c0 < B.n_in; c0 += kChunk) {
+        const int nc = std::min(kChunk, B.n_in - c0);
+        chunks.push_back(RedChunk{gb, 0, B.col_in0 + c0, nc, poff, 0});
+        poff += B.n_outlev;
+      }
+      B.n_red_in = (int)chunks.size() - B.red_in0;
+      B.red_an0 = (int)chunks.size();
+      if (B.nlev > 0)
+        for (int c0 = 0; c0 < B.n_an; c0 += kChunk) {
+          const int nc = std::min(kChunk, B.n_an - c0);
+          chunks.push_back(RedChunk{gb, 1, B.col_an0 + c0, nc, poff, 0});
+          poff += B.nlev;
+        }
+      B.n_red_an = (int)chunks.size() - B.red_an0;
     }
   }
   try {
@@ -570,6 +593,10 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
     std::vector<double> m(std::max(mean_off, 1), 0.0);
     t.d_means = dev_upload(m);
     t.nmeans = mean_off;
+    t.d_rchunk = dev_upload(chunks);
+    t.nrchunk = (int)chunks.size();
+    std::vector<double> part(std::max(poff, 1), 0.0);
+    t.d_partial = dev_upload(part);
   } catch (HipFail& e) {
     return fail(GW_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e.e));
   }
@@ -980,6 +1007,52 @@ double gw_eigen_estimate(gw_ctx* ctx, int32_t s) {
 int gw_sync(gw_ctx* ctx) {
   if (!ctx) return fail(GW_EINVAL, "null argument");
   return hip_status(hipStreamSynchronize(ctx->stream), "sync");
+}
+
+void* gw_stream(gw_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps, double* ms) {
+  if (!ctx || !phi || !ms || nsteps < 0) return fail(GW_EINVAL, "bad argument");
+  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
+  for (int q = 0; q < 6; ++q) ms[q] = 0.0;
+  CTab* tab = nullptr;
+  int rc = build_ctab(*ctx, dt / 2, tab);
+  if (rc) return rc;
+  rc = check_zstep(*ctx, dt);
+  if (rc) return rc;
+  hipEvent_t ev[7] = {};
+  for (auto& e : ev) {
+    rc = hip_status(hipEventCreate(&e), "event create");
+    if (rc) return rc;
+  }
+  const double c = dt / 2;
+  DevView v = ctx->view(tab);
+  auto rec = [&](int q) { return hip_status(hipEventRecord(ev[q], ctx->stream), "event record"); };
+  for (int n = 0; n < nsteps && rc == GW_OK; ++n) {
+    const double te = t + (n + 1) * dt;
+    if ((rc = rec(0))) break;
+    if ((rc = hip_status(launch_zstep(ctx->view(nullptr), ctx->prec, phi, dt, ctx->c3, ctx->stream), "zstep"))) break;
+    if ((rc = rec(1))) break;
+    if ((rc = hip_status(launch_line(v, ctx->prec, LINE_EXPLICIT, 1, phi, phi, c, ctx->stream), "explicit y"))) break;
+    if ((rc = rec(2))) break;
+    if ((rc = sub_step_advect(*ctx, tab, phi, te))) break;
+    if ((rc = rec(3))) break;
+    if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, phi, phi, c, ctx->stream), "x"))) break;
+    if ((rc = rec(4))) break;
+    if ((rc = sub_step_advect(*ctx, tab, phi, te + c))) break;
+    if ((rc = rec(5))) break;
+    if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, phi, phi, c, ctx->stream), "implicit y"))) break;
+    if ((rc = rec(6))) break;
+    if ((rc = hip_status(hipEventSynchronize(ev[6]), "event sync"))) break;
+    for (int q = 0; q < 6; ++q) {
+      float f = 0.0f;
+      if ((rc = hip_status(hipEventElapsedTime(&f, ev[q], ev[q + 1]), "elapsed"))) break;
+      ms[q] += f;
+    }
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
 }
 
 }  // extern "C"

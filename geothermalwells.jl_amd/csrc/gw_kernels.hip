@@ -22,6 +22,24 @@ namespace gw {
 
 #define GW_DEV __device__ __forceinline__
 
+// p + byte_off for a wave-uniform p and a 32-bit per-lane byte offset (lets
+// the compiler use the scalar-base + 32-bit vector-offset addressing form).
+template <class T>
+GW_DEV T* lane_ptr(T* p, unsigned byte_off) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(p) + byte_off);
+}
+template <class T>
+GW_DEV const T* lane_ptr(const T* p, unsigned byte_off) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + byte_off);
+}
+
+// Marks a wave-uniform 64-bit value as such (it then lives in SGPRs).
+GW_DEV int64_t uniform64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Scenario s with prefix[s] <= idx < prefix[s+1].
 GW_DEV int find_scen(const int64_t* prefix, int n, int64_t idx) {
   int lo = 0, hi = n - 1;
@@ -236,8 +254,11 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   const int seg = (L == 64) ? __builtin_amdgcn_readfirstlane(tid / 64) : tid / L;
   const int N = dir == 0 ? S_.nx : S_.ny;
   const int64_t stride = dir == 0 ? (int64_t)S_.nz * S_.ny : (int64_t)S_.nz;
-  const int kq = ll < T.nk ? ll : T.nk - 1;  // idle lanes shadow the tile's last line
-  const int64_t base = S_.off + (dir == 0 ? (int64_t)S_.nz * T.line : (int64_t)S_.nz * S_.ny * T.line) + T.k0 + kq;
+  // idle lanes shadow the tile's last line; the lane offset is a 32-bit
+  // unsigned value over a wave-uniform base so loads/stores use the
+  // scalar-base addressing form (no 64-bit address per element in VGPRs)
+  const unsigned kb = (unsigned)(ll < T.nk ? ll : T.nk - 1) * (unsigned)sizeof(TS);
+  const int64_t base = S_.off + (dir == 0 ? (int64_t)S_.nz * T.line : (int64_t)S_.nz * S_.ny * T.line) + T.k0;
 
   // stage the coefficient set, identity rows beyond N
   const double* g = v.coef + v.set_off[T.sid];
@@ -252,27 +273,40 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
 
   RT d[S];
   const int l0 = seg * S;
-  const TS* pin = in + base + (int64_t)l0 * stride;
+  const int64_t ustride = uniform64(stride);
+  const int64_t useg = uniform64(base + (int64_t)l0 * stride);
+  const TS* pin = in + useg;
   if (l0 + S <= N) {
 #pragma unroll
-    for (int r = 0; r < S; ++r) d[r] = (RT)pin[(int64_t)r * stride];
+    for (int r = 0; r < S; ++r) {
+      const TS* pr = pin + (int64_t)r * ustride;
+      d[r] = (RT)*lane_ptr(pr, kb);
+    }
   } else {
 #pragma unroll
-    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? (RT)pin[(int64_t)r * stride] : (RT)0;
+    for (int r = 0; r < S; ++r) {
+      const TS* pr = pin + (int64_t)r * ustride;
+      d[r] = (l0 + r < N) ? (RT)*lane_ptr(pr, kb) : (RT)0;
+    }
   }
   __syncthreads();
 
   line_tile_body<Cfg, TS, MODE>(sh_coef, d, sh_carry, N, seg, ll, c);
 
   if (ll < T.nk) {
-    TS* pout = out + base + (int64_t)l0 * stride;
+    TS* pout = out + useg;
     if (l0 + S <= N) {
 #pragma unroll
-      for (int r = 0; r < S; ++r) pout[(int64_t)r * stride] = (TS)d[r];
+      for (int r = 0; r < S; ++r) {
+        TS* pr = pout + (int64_t)r * ustride;
+        *lane_ptr(pr, kb) = (TS)d[r];
+      }
     } else {
 #pragma unroll
-      for (int r = 0; r < S; ++r)
-        if (l0 + r < N) pout[(int64_t)r * stride] = (TS)d[r];
+      for (int r = 0; r < S; ++r) {
+        TS* pr = pout + (int64_t)r * ustride;
+        if (l0 + r < N) *lane_ptr(pr, kb) = (TS)d[r];
+      }
     }
   }
 }
@@ -295,34 +329,46 @@ GW_DEV double block_sum(double x, double* red) {
   return r;
 }
 
-// One block per borehole.
+// Pass 1: one block per reduction chunk (up to 64 pipe columns of one
+// borehole); thread t owns level t (coalesced within each column) and sums
+// the chunk's columns in order.
 template <class TS>
-__global__ __launch_bounds__(256) void k_pipe_reduce(DevView v, const TS* __restrict__ phi) {
+__global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __restrict__ phi) {
+  const RedChunk R = v.rchunk[blockIdx.x];
+  const BhDev B = v.bh[R.bh];
+  const ScenDev S = v.sc[B.scen];
+  const int k0 = R.part == 0 ? 0 : B.klo;
+  const int nl = R.part == 0 ? B.n_outlev : B.nlev;
+  const TS* base = phi + S.off + k0;
+  for (int lev = threadIdx.x; lev < nl; lev += blockDim.x) {
+    double acc = 0.0;
+    for (int c = 0; c < R.ncol; ++c) {
+      const PipeCol pc = v.pcol[R.col0 + c];
+      acc += (double)base[((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz + lev];
+    }
+    v.partial[R.poff + lev] = acc;
+  }
+}
+
+// Pass 2: one block per borehole; chunk partials are combined in chunk order
+// and the outlet levels with a fixed tree, so the result is deterministic.
+__global__ __launch_bounds__(256) void k_pipe_final(DevView v) {
   __shared__ double red[256];
   const int b = blockIdx.x;
   const BhDev B = v.bh[b];
-  const ScenDev S = v.sc[B.scen];
   const int t = threadIdx.x;
-  // outlet: inner columns x levels k < n_outlev
   double acc = 0.0;
-  const int nout = B.n_in * B.n_outlev;
-  for (int q = t; q < nout; q += 256) {
-    const int cidx = q / B.n_outlev, k = q - cidx * B.n_outlev;
-    const PipeCol pc = v.pcol[B.col_in0 + cidx];
-    acc += (double)phi[S.off + ((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz + k];
+  for (int lev = t; lev < B.n_outlev; lev += 256) {
+    double s = 0.0;
+    for (int q = 0; q < B.n_red_in; ++q) s += v.partial[v.rchunk[B.red_in0 + q].poff + lev];
+    acc += s;
   }
   const double tot = block_sum<256>(acc, red);
-  const double tout = tot / (double)nout;
-  // annulus level means for the turnaround
-  for (int lv = 0; lv < B.nlev; ++lv) {
-    const int k = B.klo + lv;
-    double a = 0.0;
-    for (int q = t; q < B.n_an; q += 256) {
-      const PipeCol pc = v.pcol[B.col_an0 + q];
-      a += (double)phi[S.off + ((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz + k];
-    }
-    const double sum = block_sum<256>(a, red);
-    if (t == 0) v.means[B.mean_off + lv] = sum / (double)B.n_an;
+  const double tout = tot / ((double)B.n_in * (double)B.n_outlev);
+  for (int lev = t; lev < B.nlev; lev += 256) {
+    double s = 0.0;
+    for (int q = 0; q < B.n_red_an; ++q) s += v.partial[v.rchunk[B.red_an0 + q].poff + lev];
+    v.means[B.mean_off + lev] = s / (double)B.n_an;
   }
   if (t == 0) {
     v.tout[b] = tout;
@@ -441,9 +487,12 @@ hipError_t launch_line(const DevView& v, int prec, int mode, int dir, void* out,
 hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st) {
   if (v.nbh == 0) return hipSuccess;
   if (prec == 0)
-    hipLaunchKernelGGL((k_pipe_reduce<double>), dim3(v.nbh), dim3(256), 0, st, v, (const double*)phi);
+    hipLaunchKernelGGL((k_pipe_partial<double>), dim3(v.nrchunk), dim3(256), 0, st, v, (const double*)phi);
   else
-    hipLaunchKernelGGL((k_pipe_reduce<float>), dim3(v.nbh), dim3(256), 0, st, v, (const float*)phi);
+    hipLaunchKernelGGL((k_pipe_partial<float>), dim3(v.nrchunk), dim3(256), 0, st, v, (const float*)phi);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_pipe_final, dim3(v.nbh), dim3(256), 0, st, v);
   return hipGetLastError();
 }
 

@@ -27,6 +27,9 @@ struct DevView {
   const PipeCol* pcol;
   int32_t npcol;
   const Dep* dep;
+  const RedChunk* rchunk;
+  int32_t nrchunk;
+  double* partial;
   double* means;
   double* tout;                 // [nbh] outlet of the current sub-step
   double* inlet;                // [nbh] inlet value of the current sub-step

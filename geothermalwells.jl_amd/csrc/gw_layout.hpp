@@ -38,8 +38,18 @@ struct BhDev {
   int32_t mean_off;         // offset of those means in the mean buffer
   int32_t dep_off;          // departure table offset (entries), 2 entries per level
   int32_t inlet_kind;       // GW_INLET_*
+  int32_t red_in0, n_red_in;  // reduction chunks of the inner columns
+  int32_t red_an0, n_red_an;  // reduction chunks of the annulus columns
   int32_t pad;
   double inlet_value;       // ConstantInlet value or HeatExchangerInlet dT
+};
+
+// A reduction chunk: pipe columns [col0, col0+ncol) of borehole `bh`; part 0
+// sums inner-pipe columns over the outlet levels k < n_outlev, part 1 annulus
+// columns over the turnaround levels [klo, klo+nlev).  Per-level partial sums
+// go to partial[poff + level].
+struct RedChunk {
+  int32_t bh, part, col0, ncol, poff, pad;
 };
 
 // One pipe column: (i, j) of a borehole; inner-pipe columns of a borehole come

@@ -150,6 +150,15 @@ int gw_index_lists(gw_ctx *ctx, int32_t scen, int32_t *inner, int32_t *outer);
 double gw_eigen_estimate(gw_ctx *ctx, int32_t scen);
 int gw_sync(gw_ctx *ctx);
 
+/* The hipStream_t the context launches on (for callers that record events). */
+void *gw_stream(gw_ctx *ctx);
+
+/* Measurement: nsteps full steps (no graph) with HIP events between the
+ * operators, on the context's stream; ms[0..5] accumulate the time of
+ * z-step, explicit y, advection sub-step 1, implicit+explicit x, advection
+ * sub-step 2, implicit y.  Synchronises. */
+int gw_profile_steps(gw_ctx *ctx, void *phi, double t, double dt, int32_t nsteps, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,128 @@
+"""GPU: batched ragged scenarios, FP32 state, host-evaluated inlets."""
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def _kw(w):
+    return {k: w[k] for k in ("gridx", "gridy", "gridz", "materials", "boreholes", "inlet_model")}
+
+
+def _ragged_scenarios():
+    g = H.gw()
+    out = []
+    for s, (h, n_fine, zr, inl) in enumerate([(60.0, 0.01, (70.0, 2.0), "constant"),
+                                              (40.0, 0.015, (46.0, 1.5), "hx"),
+                                              (75.0, 0.008, (80.0, 2.5), "perwell")]):
+        w = H.small_case(h=h, n_fine=n_fine, nz_dz=zr, inlet=inl, layered=(s == 1))
+        out.append(w)
+    return out
+
+
+def test_batched_ragged_matches_oracle_per_scenario():
+    torch = _torch()
+    g = H.gw()
+    ws = _ragged_scenarios()
+    shapes = {(len(w["gridx"]), len(w["gridy"]), len(w["gridz"])) for w in ws}
+    assert len(shapes) == len(ws)  # really ragged
+    cache = g.create_batch_cache([_kw(w) for w in ws])
+    u = cache.new_state()
+    refs = []
+    for s, w in enumerate(ws):
+        orc = H.oracle_for(w)
+        u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+        cache.scenario_view(u, s).copy_(torch.from_numpy(u0))
+        refs.append((orc, u0))
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, 3))
+    T = cache.T_outlet
+    for s, (orc, u0) in enumerate(refs):
+        ref, hist, _ = orc.run(u0, 0.0, 60.0, 3, cache.rock2_c3)
+        got = cache.scenario_view(u, s).cpu().numpy()
+        assert H.rel_err(got, ref) <= 1e-10, s
+        assert abs(T[s] - hist[-1][0]) <= 1e-6, s
+
+
+def test_fp32_state_outlet_error():
+    """FP32 field / stencil / advection with FP64 elimination and reductions:
+    the outlet stays within 1e-3 K of the FP64 oracle."""
+    torch = _torch()
+    g = H.gw()
+    w = H.small_case(nbh=2, inlet="hx")
+    orc = H.oracle_for(w)
+    u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+    cache = g.create_cache(precision="f32", **_kw(w))
+    u = torch.from_numpy(u0).to(device="cuda", dtype=torch.float32).reshape(-1).contiguous()
+    n = 10
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, n))
+    ref, hist, _ = orc.run(u0, 0.0, 60.0, n, cache.rock2_c3)
+    got = u.view(orc.shape).double().cpu().numpy()
+    assert H.rel_err(got, ref) <= 1e-5
+    assert np.abs(cache.T_outlet - hist[-1]).max() <= 1e-3
+
+
+def test_host_custom_inlet_matches_table():
+    """CustomInlet evaluated on the host each sub-step gives the same result as
+    the equivalent device-side per-well table."""
+    torch = _torch()
+    g = H.gw()
+    w = H.small_case(nbh=2, inlet="perwell")
+    vals = w["inlet_model"].values
+    w_host = dict(w, inlet_model=g.CustomInlet(lambda bh, T_out, t: vals[bh - 1]))
+    outs = []
+    for ww in (w, w_host):
+        cache = g.create_cache(**_kw(ww))
+        orc = H.oracle_for(w)
+        u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+        u = torch.from_numpy(u0).cuda().reshape(-1).contiguous()
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, 3))
+        outs.append((u.cpu().numpy(), cache.T_outlet))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_time_dependent_custom_inlet():
+    """A time-dependent CustomInlet sees t_{n+1} and t_{n+1} + dt/2 (src/solver.jl:447,458)."""
+    torch = _torch()
+    g = H.gw()
+    seen = []
+
+    def f(bh, T_out, t):
+        seen.append(t)
+        return 20.0 + 0.01 * t
+
+    w = H.small_case(inlet="constant")
+    w = dict(w, inlet_model=g.CustomInlet(f))
+    cache = g.create_cache(**_kw(w))
+    orc = H.oracle_for(H.small_case())
+    u0 = orc.initial_condition(2.29, 0.35)
+    u = torch.from_numpy(u0).cuda().reshape(-1).contiguous()
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, 2))
+    assert seen == [60.0, 90.0, 120.0, 150.0]
+
+
+def test_degenerate_inputs_are_errors():
+    """Configurations for which the reference produces NaN return GW_EINVAL."""
+    torch = _torch()
+    g = H.gw()
+    w = H.small_case()
+    bh0 = w["boreholes"][0]
+    no_flow = g.Borehole(*bh0.args[:8], 0.0, 0.0)
+    cache = g.create_cache(**dict(_kw(w), boreholes=(no_flow,)))
+    u = cache.new_state().fill_(10.0)
+    with pytest.raises(g._abi.GeoWellsError):
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, 1))
+    coarse = g.create_adaptive_grid_1d(xmin=-5.0, xmax=5.0, dx_fine=0.5, growth_factor=2.0, dx_max=2.0,
+                                       boreholes=(bh0,), direction="x") + 0.013
+    cache = g.create_cache(**dict(_kw(w), gridx=coarse, gridy=coarse))
+    u = cache.new_state().fill_(10.0)
+    with pytest.raises(g._abi.GeoWellsError, match="inner-pipe"):
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, 1))
