@@ -3,8 +3,10 @@
 Workload: the 3x3 borehole array (401 x 401 x 1002 cells, FP64, dt 300 s,
 per-well inlets) -- the plain-FP64 GPU configuration of the metric.  A step is
 one full dt: ROCK2 z-step, explicit-y, advection, implicit-x + explicit-x,
-advection, implicit-y.  The state stays resident in HBM; each step is one
-captured hipGraph replay.
+advection, implicit-y.  The state stays resident in HBM.  The library runs
+the steps as a fused pipeline (the implicit y of one step shares a kernel
+with the next step's z-step and explicit y), replayed as one captured
+hipGraph per step.
 
 Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits: this
 build measures one GPU.
@@ -25,7 +27,11 @@ import __graft_entry__ as entry  # noqa: E402
 
 HBM_PEAK = 8.0e12  # B/s, MI355X
 METRIC = "cell-steps/s per GPU, plain FP64 vs batched/FP32 at ≤1e-3 K outlet err; %HBM BW"
-OPS = ["zstep", "explicit_y", "advect_1", "implicit_explicit_x", "advect_2", "implicit_y"]
+# kernels of the fused step pipeline (gw_profile_steps): per step
+# advect_1, implicit_explicit_x, advect_2 and implicit y fused with the next
+# step's z-step and explicit y ("yzy"); the first step's z-step + explicit y
+# ("zy") and the last step's implicit y ("implicit_y") run once.
+OPS = ["zy", "advect_1", "implicit_explicit_x", "advect_2", "yzy", "implicit_y"]
 
 
 def pipe_bytes(cache, w, s):
@@ -118,9 +124,11 @@ def main():
     ms = (ctypes.c_double * 6)()
     t0 = (args.warmup + args.steps) * dt
     gw._abi.check(lib.gw_profile_steps(cache.ctx, ptr, t0, dt, args.profile_steps, ms))
-    per_op = {OPS[q]: ms[q] / args.profile_steps for q in range(6)}
+    n_p = args.profile_steps
+    launches = [1, n_p, n_p, n_p, n_p - 1, 1]
+    per_op = {OPS[q]: ms[q] / launches[q] for q in range(6) if launches[q] > 0}
     s = 8
-    field_ops = ["zstep", "explicit_y", "implicit_explicit_x", "implicit_y"]
+    field_ops = ["implicit_explicit_x", "yzy"]
     dom = max(field_ops, key=lambda k: per_op[k])
     dom_bytes = cells * 2 * s  # one read + one write of the state per launch
     achieved = dom_bytes / (per_op[dom] * 1e-3)
@@ -128,7 +136,7 @@ def main():
     Bp = pipe_bytes(cache, w, s)
     steps_per_s = args.steps / wall
     step_bw = value * B_cs + steps_per_s * Bp
-    fused_bw = value * 8 * s + steps_per_s * Bp
+    fused_bw = value * 4 * s + steps_per_s * Bp
 
     out = {
         "metric": METRIC,
@@ -149,7 +157,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": per_op[dom]},
         "step_roofline": {"B_cs_5sweep": B_cs, "pipe_bytes_per_step": Bp,
-                          "frac_5sweep": step_bw / HBM_PEAK, "frac_4sweep_as_run": fused_bw / HBM_PEAK},
+                          "frac_5sweep": step_bw / HBM_PEAK, "frac_2sweep_as_run": fused_bw / HBM_PEAK},
         "per_op_ms": per_op,
         "event_ms_per_step": ev_ms / args.steps,
         "outlet_T": [float(x) for x in T_out],

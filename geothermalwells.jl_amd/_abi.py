@@ -80,6 +80,8 @@ SIGNATURES = {
     "gw_eigen_estimate": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int32]),
     "gw_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "gw_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "gw_line_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int32, ctypes.c_int32]),
     "gw_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int32, c_double_p]),
 }

@@ -223,7 +223,7 @@ struct gw_ctx {
   std::vector<void*> allocs;
   ScenDev* d_sc = nullptr;
   int64_t *d_cell_prefix = nullptr, *d_col_prefix = nullptr;
-  LineTile *d_xtiles = nullptr, *d_ytiles = nullptr;
+  LineTile *d_xtiles = nullptr, *d_ytiles = nullptr, *d_htiles = nullptr;
   uint16_t* d_cls = nullptr;
   int32_t* d_zone = nullptr;
   double* d_zcoef = nullptr;
@@ -231,13 +231,17 @@ struct gw_ctx {
   PipeCol* d_pcol = nullptr;
   double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
   int nx_max = 0, ny_max = 0, nz_max = 0, countz_max = 0;
-  int64_t cells_total = 0, cols_total = 0, xtiles_total = 0, ytiles_total = 0;
+  int64_t cells_total = 0, cols_total = 0, xtiles_total = 0, ytiles_total = 0, htiles_total = 0;
+  bool fused_ok = false;  // fused z-step line kernels usable (few enough column classes)
   std::map<double, CTab> ctabs;
   bool host_inlet = false;
   // captured step graph
-  hipGraphExec_t graph_exec = nullptr;
+  hipGraphExec_t graph_exec = nullptr;   // step body, state phi -> buf2
+  hipGraphExec_t graph_exec2 = nullptr;  // step body, state buf2 -> phi
   void* graph_phi = nullptr;
   double graph_dt = 0;
+  // second field buffer of the fused pipeline (its halo kernels write out of place)
+  void* buf2 = nullptr;
 
   template <class T>
   T* up(const std::vector<T>& h) {
@@ -247,6 +251,8 @@ struct gw_ctx {
   }
   ~gw_ctx() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph_exec2) (void)hipGraphExecDestroy(graph_exec2);
+    if (buf2) (void)hipFree(buf2);
     for (auto& kv : ctabs) {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
@@ -259,7 +265,7 @@ struct gw_ctx {
     DevView v{};
     v.sc = d_sc; v.nscen = (int)sc.size();
     v.cell_prefix = d_cell_prefix; v.col_prefix = d_col_prefix;
-    v.xtiles = d_xtiles; v.ytiles = d_ytiles;
+    v.xtiles = d_xtiles; v.ytiles = d_ytiles; v.htiles = d_htiles;
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
     if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial; }
@@ -267,7 +273,7 @@ struct gw_ctx {
     v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
     v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
     v.cells_total = cells_total; v.cols_total = cols_total;
-    v.xtiles_total = xtiles_total; v.ytiles_total = ytiles_total;
+    v.xtiles_total = xtiles_total; v.ytiles_total = ytiles_total; v.htiles_total = htiles_total;
     return v;
   }
 };
@@ -403,23 +409,49 @@ void build_line_sets(gw_ctx& ctx, Scen& s, std::unordered_map<std::string, int>&
   }
 }
 
-// Line tiles of one scenario and direction: 64-wide k ranges of every line,
-// split where the coefficient set changes (material zone boundaries in z).
-void line_tiles(const Scen& s, int si, int dir, std::vector<LineTile>& out) {
+// Line tiles of one scenario and direction.  Each tile is a 64-lane k window
+// whose lanes meet at most kMaxSets coefficient sets.  Plain tiles output all
+// their lanes; halo tiles (halo = 3, for the fused z-step) output lanes
+// halo .. 63-halo, the others only feed the z-stencil of the outputs.
+constexpr int kMaxSets = 3;
+
+void line_tiles(const Scen& s, int si, int dir, int halo, std::vector<LineTile>& out) {
   const int nlines = dir == 0 ? s.ny : s.nx;
   const std::vector<int32_t>& ids = dir == 0 ? s.xset : s.yset;
-  for (int line = 0; line < nlines; ++line)
-    for (int kc = 0; kc < s.nz; kc += kTileL) {
-      const int kend = std::min(s.nz, kc + kTileL);
-      int k = kc;
-      while (k < kend) {
-        const int sid = ids[(size_t)line * s.nzone + s.zone[k]];
-        int k2 = k + 1;
-        while (k2 < kend && ids[(size_t)line * s.nzone + s.zone[k2]] == sid) ++k2;
-        out.push_back(LineTile{si, line, k, k2 - k, sid, 0, 0, 0});
-        k = k2;
+  const int width = kTileL - 2 * halo;
+  for (int line = 0; line < nlines; ++line) {
+    auto sid_at = [&](int k) {
+      const int kc = std::min(std::max(k, 0), s.nz - 1);
+      return ids[(size_t)line * s.nzone + s.zone[kc]];
+    };
+    int k0 = 0;
+    while (k0 < s.nz) {
+      // widest output range [k0, k0+n) whose needed lanes [k0-halo, k0+n+halo)
+      // use at most kMaxSets sets
+      std::vector<int> sets;
+      int n = 0;
+      for (int kk = k0 - halo; kk < k0 + halo; ++kk) {
+        const int q = sid_at(kk);
+        if (std::find(sets.begin(), sets.end(), q) == sets.end()) sets.push_back(q);
       }
+      while (n < width && k0 + n < s.nz) {
+        const int q = sid_at(k0 + n + halo);
+        if (std::find(sets.begin(), sets.end(), q) == sets.end()) {
+          if ((int)sets.size() == kMaxSets) break;
+          sets.push_back(q);
+        }
+        ++n;
+      }
+      if (n == 0 || (int)sets.size() > kMaxSets)
+        throw std::runtime_error("more than 3 material zones within a 7-level z window");
+      LineTile t{};
+      t.scen = si; t.line = line; t.kw = k0 - halo; t.ko0 = k0; t.ko1 = k0 + n;
+      t.nset = (int)sets.size();
+      for (int q = 0; q < 3; ++q) t.set[q] = q < t.nset ? sets[q] : -1;
+      out.push_back(t);
+      k0 += n;
     }
+  }
 }
 
 // create_advection_index_lists  (src/cache.jl:9-44)
@@ -472,24 +504,19 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   for (size_t q = 0; q < ctx.sets.size(); ++q) {
     const LineSet& L = ctx.sets[q];
     const int N = L.N;
-    double* fw = &coef[set_off[q]];
-    double* bd = fw + N;
-    double* bu = fw + 2 * N;
-    double* ea = fw + 3 * N;
-    double* eb = fw + 4 * N;
+    double* bd = &coef[set_off[q]];
+    double* ea = bd + N;
+    double* eb = bd + 2 * N;
+    // (I - cA): lower_l = -c eb_l, diag_l = 1 + c (ea_l + eb_l), upper_l = -c ea_l
     double piv = 1.0 + c * (L.ea[0] + L.eb[0]);
-    fw[0] = 0.0;
     for (int l = 1; l < N; ++l) {
       const double lower = -c * L.eb[l];
       const double upper_prev = -c * L.ea[l - 1];
       const double w = lower / piv;
       bd[l - 1] = 1.0 / piv;
-      bu[l - 1] = upper_prev / piv;
-      fw[l] = w;
       piv = (1.0 + c * (L.ea[l] + L.eb[l])) - w * upper_prev;
     }
     bd[N - 1] = 1.0 / piv;
-    bu[N - 1] = 0.0;
     for (int l = 0; l < N; ++l) { ea[l] = L.ea[l]; eb[l] = L.eb[l]; }
   }
   // departure tables
@@ -661,17 +688,17 @@ int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
   if (rc) return rc;
   DevView v = ctx.view(tab);
   // explicit y  (src/solver.jl:444)
-  rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, phi, phi, c, ctx.stream), "explicit y");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, phi, phi, c, 0.0, 0.0, ctx.stream), "explicit y");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, phi, t);  // :447
   if (rc) return rc;
   // implicit x then explicit x  (:450, :455)
-  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, phi, phi, c, ctx.stream), "implicit+explicit x");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, phi, phi, c, 0.0, 0.0, ctx.stream), "implicit+explicit x");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, phi, t + dt / 2);  // :458
   if (rc) return rc;
   // implicit y  (:461)
-  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, phi, phi, c, ctx.stream), "implicit y");
+  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, phi, phi, c, 0.0, 0.0, ctx.stream), "implicit y");
 }
 
 // The z-step applies R(dt A_z) = 1 + z + z^2/2 + c3 z^3 at every ROCK2 degree:
@@ -777,7 +804,7 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     }
     std::vector<ScenDev> sd;
     std::vector<int64_t> cellp{0}, colp{0};
-    std::vector<LineTile> xt, yt;
+    std::vector<LineTile> xt, yt, ht;
     std::vector<uint16_t> cls;
     std::vector<int32_t> zone, setid;
     std::vector<double> zcoef, inlet_tab;
@@ -796,8 +823,9 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       const int64_t ncell = (int64_t)s.nx * s.ny * s.nz;
       cellp.push_back(cellp.back() + ncell);
       colp.push_back(colp.back() + (int64_t)s.nx * s.ny);
-      line_tiles(s, (int)sd.size() - 1, 0, xt);
-      line_tiles(s, (int)sd.size() - 1, 1, yt);
+      line_tiles(s, (int)sd.size() - 1, 0, 0, xt);
+      line_tiles(s, (int)sd.size() - 1, 1, 0, yt);
+      line_tiles(s, (int)sd.size() - 1, 1, 3, ht);
       for (size_t b = 0; b < s.bh.size(); ++b)
         inlet_tab.push_back(s.inlet.kind == GW_INLET_PER_WELL ? s.per_well[b] : 0.0);
       gbh += (int)s.bh.size();
@@ -810,11 +838,15 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     ctx->cols_total = colp.back();
     ctx->xtiles_total = (int64_t)xt.size();
     ctx->ytiles_total = (int64_t)yt.size();
+    ctx->htiles_total = (int64_t)ht.size();
     ctx->d_sc = ctx->up(sd);
     ctx->d_cell_prefix = ctx->up(cellp);
     ctx->d_col_prefix = ctx->up(colp);
     ctx->d_xtiles = ctx->up(xt);
     ctx->d_ytiles = ctx->up(yt);
+    ctx->d_htiles = ctx->up(ht);
+    ctx->fused_ok = true;
+    for (auto& s : ctx->sc) ctx->fused_ok = ctx->fused_ok && s.ncls <= 16;
     ctx->d_cls = ctx->up(cls);
     ctx->d_zone = ctx->up(zone);
     ctx->d_zcoef = ctx->up(zcoef);
@@ -871,53 +903,141 @@ int gw_zstep(gw_ctx* ctx, void* phi, double dt) {
   return zstep(*ctx, phi, dt);
 }
 
-int gw_step(gw_ctx* ctx, void* phi, double t, double dt) {
-  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
-  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
-  int rc = zstep(*ctx, phi, dt);
+}  // extern "C"
+
+namespace {
+
+// Fused step pipeline.  Steps n .. n+k-1 are the launch sequence
+//   ZY                                    (z-step + explicit y of step n)
+//   [adv, X, adv, YZY] for all but the last step
+//   adv, X, adv, Yimpl                    (the last step's implicit y)
+// where X = implicit x + explicit x and YZY = implicit y of one step, then
+// the z-step and explicit y of the next; the field is read and written twice
+// per step instead of four times, with the arithmetic of the separate
+// operators.  ZY and YZY read a 3-level halo of neighbouring k-windows, so
+// they write out of place: the state alternates between the caller's field
+// and the context's second buffer (ZY: phi -> buf2, the step ending in YZY
+// or Yimpl moves it back), and the last step's implicit y always lands in phi.
+int ensure_buf2(gw_ctx& ctx) {
+  if (ctx.buf2) return GW_OK;
+  const size_t bytes = (size_t)ctx.cells_total * (ctx.prec == GW_F64 ? 8 : 4);
+  if (hipMalloc(&ctx.buf2, bytes) != hipSuccess) {
+    ctx.buf2 = nullptr;
+    return fail(GW_ENOMEM, "cannot allocate the second field buffer of the fused step pipeline");
+  }
+  return GW_OK;
+}
+
+int fused_zy(gw_ctx& ctx, const CTab* tab, const void* in, void* out, double dt) {
+  return hip_status(launch_line(ctx.view(tab), ctx.prec, LINE_ZY, 1, out, in, dt / 2, dt, ctx.c3, ctx.stream),
+                    "z-step + explicit y");
+}
+
+// One step's advection/x part (in place on `cur`) and its closing y kernel
+// from `cur` into `next` (YZY, or the plain implicit y when `last`).
+int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end, double dt, bool last) {
+  const double c = dt / 2;
+  DevView v = ctx.view(tab);
+  int rc = sub_step_advect(ctx, tab, cur, t_end);
   if (rc) return rc;
-  return adi_adv(*ctx, phi, t + dt, dt);
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, cur, cur, c, 0.0, 0.0, ctx.stream), "x");
+  if (rc) return rc;
+  rc = sub_step_advect(ctx, tab, cur, t_end + c);
+  if (rc) return rc;
+  if (last)
+    return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, next, cur, c, 0.0, 0.0, ctx.stream), "implicit y");
+  return hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, next, cur, c, dt, ctx.c3, ctx.stream),
+                    "implicit y + z-step + explicit y");
+}
+
+int prepare_steps(gw_ctx& ctx, double dt, CTab*& tab) {
+  int rc = build_ctab(ctx, dt / 2, tab);
+  if (rc) return rc;
+  rc = check_zstep(ctx, dt);
+  if (rc) return rc;
+  return ensure_buf2(ctx);
+}
+
+int capture_body(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t, double dt, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  int rc = hip_status(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  if (rc) return rc;
+  const int rc1 = fused_rest(ctx, tab, cur, next, t, dt, false);
+  const hipError_t e2 = hipStreamEndCapture(ctx.stream, &g);
+  if (rc1) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc1;
+  }
+  if ((rc = hip_status(e2, "end capture"))) return rc;
+  rc = hip_status(hipGraphInstantiate(out, g, nullptr, nullptr, 0), "graph instantiate");
+  (void)hipGraphDestroy(g);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gw_step(gw_ctx* ctx, void* phi, double t, double dt) {
+  return gw_steps(ctx, phi, t, dt, 1);
 }
 
 int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
   if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
   if (!(dt > 0) || nsteps < 0) return fail(GW_EINVAL, "dt must be positive and nsteps >= 0");
   if (nsteps == 0) return GW_OK;
-  if (ctx->host_inlet) {
+  if (!ctx->fused_ok) {
     for (int n = 0; n < nsteps; ++n) {
-      const int rc = gw_step(ctx, phi, t + n * dt, dt);
+      int rc = zstep(*ctx, phi, dt);
       if (rc) return rc;
+      if ((rc = adi_adv(*ctx, phi, t + (n + 1) * dt, dt))) return rc;
     }
     return GW_OK;
   }
-  // Every launch of a step is independent of t for device-side inlets, so
-  // one captured graph is replayed.
-  if (!ctx->graph_exec || ctx->graph_phi != phi || ctx->graph_dt != dt) {
+  CTab* tab = nullptr;
+  int rc = prepare_steps(*ctx, dt, tab);
+  if (rc) return rc;
+  void* B = ctx->buf2;
+  rc = fused_zy(*ctx, tab, phi, B, dt);
+  if (rc) return rc;
+  // step k (1-based) works on B when k is odd, on phi when k is even
+  const bool use_graph = !ctx->host_inlet && nsteps > 2;
+  if (use_graph && (!ctx->graph_exec || ctx->graph_phi != phi || ctx->graph_dt != dt)) {
     if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
-    CTab* tab = nullptr;
-    int rc = build_ctab(*ctx, dt / 2, tab);  // host work outside the capture
-    if (rc) return rc;
-    rc = check_zstep(*ctx, dt);
-    if (rc) return rc;
-    hipGraph_t g = nullptr;
-    rc = hip_status(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-    if (rc) return rc;
-    const int rc1 = gw_step(ctx, phi, t, dt);
-    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
-    if (rc1) { if (g) (void)hipGraphDestroy(g); return rc1; }
-    rc = hip_status(e2, "end capture");
-    if (rc) return rc;
-    rc = hip_status(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0), "graph instantiate");
-    (void)hipGraphDestroy(g);
-    if (rc) return rc;
+    if (ctx->graph_exec2) { (void)hipGraphExecDestroy(ctx->graph_exec2); ctx->graph_exec2 = nullptr; }
+    if ((rc = capture_body(*ctx, tab, B, phi, t + dt, dt, &ctx->graph_exec2))) return rc;  // B -> phi
+    if ((rc = capture_body(*ctx, tab, phi, B, t + dt, dt, &ctx->graph_exec))) return rc;   // phi -> B
     ctx->graph_phi = phi;
     ctx->graph_dt = dt;
   }
-  for (int n = 0; n < nsteps; ++n) {
-    const int rc = hip_status(hipGraphLaunch(ctx->graph_exec, ctx->stream), "graph launch");
+  for (int k = 1; k < nsteps; ++k) {
+    const bool odd = (k & 1) != 0;
+    void* cur = odd ? B : phi;
+    void* next = odd ? phi : B;
+    if (use_graph)
+      rc = hip_status(hipGraphLaunch(odd ? ctx->graph_exec2 : ctx->graph_exec, ctx->stream), "graph launch");
+    else
+      rc = fused_rest(*ctx, tab, cur, next, t + k * dt, dt, false);
     if (rc) return rc;
   }
-  return GW_OK;
+  void* cur = (nsteps & 1) ? B : phi;
+  return fused_rest(*ctx, tab, cur, phi, t + nsteps * dt, dt, true);
+}
+
+int gw_line_op(gw_ctx* ctx, void* out, const void* in, double c, double dt_z, int32_t dir, int32_t mode) {
+  if (!ctx || !out || !in || (dir != 0 && dir != 1) || mode < 0 || mode > 4) return fail(GW_EINVAL, "bad argument");
+  if (mode >= LINE_ZY) {
+    if (dir != 1) return fail(GW_EINVAL, "the fused z-step modes run on y-lines");
+    if (out == in) return fail(GW_EINVAL, "the fused z-step modes need out != in");
+    if (!ctx->fused_ok) return fail(GW_ESTATE, "too many column classes for the fused z-step kernels");
+    const int rc = check_zstep(*ctx, dt_z);
+    if (rc) return rc;
+  }
+  CTab* tab = nullptr;
+  int rc = build_ctab(*ctx, c, tab);
+  if (rc) return rc;
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, out, in, c, dt_z, ctx->c3, ctx->stream),
+                    "line operator");
 }
 
 int gw_outlet(gw_ctx* ctx, double* T_out) {
@@ -953,7 +1073,7 @@ int gw_explicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, out, in, c, ctx->stream), "explicit");
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, out, in, c, 0.0, 0.0, ctx->stream), "explicit");
 }
 
 int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
@@ -961,7 +1081,7 @@ int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, out, in, c, ctx->stream), "implicit");
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, out, in, c, 0.0, 0.0, ctx->stream), "implicit");
 }
 
 int gw_advect(gw_ctx* ctx, void* phi, double c, double t) {
@@ -1015,41 +1135,44 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
   if (!ctx || !phi || !ms || nsteps < 0) return fail(GW_EINVAL, "bad argument");
   if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
   for (int q = 0; q < 6; ++q) ms[q] = 0.0;
+  if (!ctx->fused_ok) return fail(GW_ESTATE, "profiling follows the fused step pipeline, unavailable here");
+  if (nsteps == 0) return GW_OK;
   CTab* tab = nullptr;
-  int rc = build_ctab(*ctx, dt / 2, tab);
+  int rc = prepare_steps(*ctx, dt, tab);
   if (rc) return rc;
-  rc = check_zstep(*ctx, dt);
-  if (rc) return rc;
-  hipEvent_t ev[7] = {};
-  for (auto& e : ev) {
-    rc = hip_status(hipEventCreate(&e), "event create");
-    if (rc) return rc;
-  }
+  hipEvent_t ev[5] = {};
+  for (auto& e : ev)
+    if ((rc = hip_status(hipEventCreate(&e), "event create"))) return rc;
   const double c = dt / 2;
   DevView v = ctx->view(tab);
   auto rec = [&](int q) { return hip_status(hipEventRecord(ev[q], ctx->stream), "event record"); };
-  for (int n = 0; n < nsteps && rc == GW_OK; ++n) {
-    const double te = t + (n + 1) * dt;
-    if ((rc = rec(0))) break;
-    if ((rc = hip_status(launch_zstep(ctx->view(nullptr), ctx->prec, phi, dt, ctx->c3, ctx->stream), "zstep"))) break;
-    if ((rc = rec(1))) break;
-    if ((rc = hip_status(launch_line(v, ctx->prec, LINE_EXPLICIT, 1, phi, phi, c, ctx->stream), "explicit y"))) break;
-    if ((rc = rec(2))) break;
-    if ((rc = sub_step_advect(*ctx, tab, phi, te))) break;
-    if ((rc = rec(3))) break;
-    if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, phi, phi, c, ctx->stream), "x"))) break;
-    if ((rc = rec(4))) break;
-    if ((rc = sub_step_advect(*ctx, tab, phi, te + c))) break;
-    if ((rc = rec(5))) break;
-    if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, phi, phi, c, ctx->stream), "implicit y"))) break;
-    if ((rc = rec(6))) break;
-    if ((rc = hip_status(hipEventSynchronize(ev[6]), "event sync"))) break;
-    for (int q = 0; q < 6; ++q) {
-      float f = 0.0f;
-      if ((rc = hip_status(hipEventElapsedTime(&f, ev[q], ev[q + 1]), "elapsed"))) break;
-      ms[q] += f;
+  auto acc = [&](int a, int b, int q) -> int {
+    float f = 0.0f;
+    const int r = hip_status(hipEventElapsedTime(&f, ev[a], ev[b]), "elapsed");
+    ms[q] += f;
+    return r;
+  };
+  do {
+    void* B = ctx->buf2;
+    if ((rc = rec(0)) || (rc = fused_zy(*ctx, tab, phi, B, dt)) || (rc = rec(1))) break;
+    if ((rc = hip_status(hipEventSynchronize(ev[1]), "event sync")) || (rc = acc(0, 1, 0))) break;
+    for (int n = 0; n < nsteps && rc == GW_OK; ++n) {
+      const double te = t + (n + 1) * dt;
+      const bool last = n + 1 == nsteps;
+      void* cur = (n & 1) ? phi : B;
+      void* next = last ? phi : ((n & 1) ? B : phi);
+      if ((rc = rec(0)) || (rc = sub_step_advect(*ctx, tab, cur, te)) || (rc = rec(1))) break;
+      if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, cur, cur, c, 0.0, 0.0, ctx->stream),
+                           "x")) || (rc = rec(2)))
+        break;
+      if ((rc = sub_step_advect(*ctx, tab, cur, te + c)) || (rc = rec(3))) break;
+      rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, next, cur, c, 0.0, 0.0, ctx->stream), "y")
+                : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, next, cur, c, dt, ctx->c3, ctx->stream), "yzy");
+      if (rc || (rc = rec(4))) break;
+      if ((rc = hip_status(hipEventSynchronize(ev[4]), "event sync"))) break;
+      if ((rc = acc(0, 1, 1)) || (rc = acc(1, 2, 2)) || (rc = acc(2, 3, 3)) || (rc = acc(3, 4, last ? 5 : 4))) break;
     }
-  }
+  } while (false);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   return rc;

@@ -21,6 +21,9 @@
 namespace gw {
 
 #define GW_DEV __device__ __forceinline__
+#ifndef GW_SCHED_FENCE
+#define GW_SCHED_FENCE() ((void)0)
+#endif
 
 // p + byte_off for a wave-uniform p and a 32-bit per-lane byte offset (lets
 // the compiler use the scalar-base + 32-bit vector-offset addressing form).
@@ -39,6 +42,23 @@ GW_DEV int64_t uniform64(int64_t x) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
+
+// Value of lane-1 (wave_shr:1) / lane+1 (wave_shl:1) across the whole wave;
+// lane 0 (resp. 63) reads 0.
+GW_DEV uint32_t dpp_shr1(uint32_t x) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x138, 0xf, 0xf, true); }
+GW_DEV uint32_t dpp_shl1(uint32_t x) { return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x130, 0xf, 0xf, true); }
+GW_DEV double wave_shr1(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint64_t r = (uint64_t)dpp_shr1((uint32_t)b) | ((uint64_t)dpp_shr1((uint32_t)(b >> 32)) << 32);
+  return __builtin_bit_cast(double, r);
+}
+GW_DEV double wave_shl1(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint64_t r = (uint64_t)dpp_shl1((uint32_t)b) | ((uint64_t)dpp_shl1((uint32_t)(b >> 32)) << 32);
+  return __builtin_bit_cast(double, r);
+}
+GW_DEV float wave_shr1(float v) { return __builtin_bit_cast(float, dpp_shr1(__builtin_bit_cast(uint32_t, v))); }
+GW_DEV float wave_shl1(float v) { return __builtin_bit_cast(float, dpp_shl1(__builtin_bit_cast(uint32_t, v))); }
 
 // Scenario s with prefix[s] <= idx < prefix[s+1].
 GW_DEV int find_scen(const int64_t* prefix, int n, int64_t idx) {
@@ -138,137 +158,110 @@ __global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, 
 }
 
 // ---------------------------------------------------------------------------
-// Line kernel.  A tile is up to L adjacent k-lines (k0 .. k0+nk-1) of one
-// x-line (fixed j) or y-line (fixed i) that share one coefficient set (the
-// host splits 64-wide k ranges at material zone boundaries).  The block has
-// NW waves; a wave's 64 lanes cover L lines x (64/L) segments, so the block
-// covers G = NW*64/L segments of S consecutive line elements (G*S >= N).
-// Each lane keeps its segment in registers.  The Thomas recurrences run
-// segment-locally with zero carry and are then corrected by the carried value
-// times the running product of the multipliers; the carries are a short scan
-// over the segments in LDS.  Loads and stores of one element index cover L
-// consecutive k values: coalesced.  The coefficient set is staged in LDS,
-// padded to G*S with identity rows so the element loops carry no branches.
+// Line kernel.  A tile is a window of 64 adjacent k-lines (k = kw .. kw+63,
+// one per lane) of one x-line (fixed j) or y-line (fixed i); its outputs are
+// the lanes with ko0 <= k < ko1.  The block has NW waves; wave w holds line
+// elements [w*S, w*S + S) of its lane's line in registers (NW*S >= N).
+// Loads and stores of one element index are 64 consecutive k values:
+// coalesced.  The Thomas recurrences run segment-locally from zero carry and
+// are then corrected by the carried value times the running product of the
+// multipliers; the carries are an NW-step scan in LDS.  The lanes of a window
+// use at most three coefficient sets (one per material zone met in z), staged
+// in LDS and padded with identity rows so the element loops have no branches.
+//
+// Modes (the reference's operators on the same direction, src/solver.jl):
+//   EXPLICIT            u <- (I + cA) u                               (:34-58)
+//   IMPLICIT            u <- (I - cA)^-1 u                            (:111-232)
+//   IMPLICIT_EXPLICIT   u <- (I + cA)(I - cA)^-1 u   x-lines, :450 + :455
+//   ZY                  u <- (I + cA_y) R(dt A_z) u   next step's z-step and
+//                       explicit y (:23-31 under ROCK2, then :444)
+//   YZY                 u <- (I + cA_y) R(dt A_z) (I - cA_y)^-1 u      (:461, then
+//                       the next step's z-step and explicit y)
+// ZY / YZY run on y-lines with a 3-lane halo in k: the z-stencil is applied
+// three times across lanes (shuffles), so only lanes 3..60 of a window are
+// outputs; the halo lanes re-solve their own lines.
 // ---------------------------------------------------------------------------
-template <int L_, int NW_, int S_>
+template <int NW_, int S_>
 struct LineCfg {
-  static constexpr int L = L_, NW = NW_, S = S_;
-  static constexpr int SPW = 64 / L_;     // segments per wave
-  static constexpr int G = NW_ * SPW;     // segments per tile
-  static constexpr int NMAX = G * S_;     // longest line
+  static constexpr int NW = NW_, S = S_;
+  static constexpr int NMAX = NW_ * S_;     // longest line
   static constexpr int THREADS = NW_ * 64;
 };
 
-template <class Cfg, class TS, int MODE, class RT>
-GW_DEV void line_tile_body(const double* cf, RT (&d)[Cfg::S], double* sh_carry, int N, int seg,
-                           int ll, double c) {
-  constexpr int S = Cfg::S, G = Cfg::G, L = Cfg::L, NM = Cfg::NMAX;
-  const double* fw = cf;
-  const double* bd = cf + NM;
-  const double* bu = cf + 2 * NM;
-  const double* ea = cf + 3 * NM;
-  const double* eb = cf + 4 * NM;
-  const int l0 = seg * S;
-  double* fy = sh_carry;              // [G][L] forward local end value
-  double* fP = sh_carry + G * L;      // [G][L] forward multiplier product
-  double* bx = sh_carry + 2 * G * L;  // [G][L] backward local start value
-  double* bQ = sh_carry + 3 * G * L;  // [G][L] backward multiplier product
-  double* xf = sh_carry + 4 * G * L;  // [G][L] first element of the segment
-  double* xl = sh_carry + 5 * G * L;  // [G][L] last element of the segment
+constexpr int kSlots = 3;      // coefficient sets per window
+constexpr int kZcMax = 16;     // column classes staged for the fused z-step
 
-  if (MODE != LINE_EXPLICIT) {
-    // forward elimination y_l = b_l - fw_l y_{l-1}, segment-local
-    double y = 0.0, P = 1.0;
-#pragma unroll
-    for (int r = 0; r < S; ++r) {
-      const double w = fw[l0 + r];
-      y = (double)d[r] - w * y;
-      P = -w * P;
-      d[r] = y;
-    }
-    fy[seg * L + ll] = y;
-    fP[seg * L + ll] = P;
-    __syncthreads();
-    double C = 0.0;
-    for (int g = 0; g < seg; ++g) C = fy[g * L + ll] + fP[g * L + ll] * C;
-    P = 1.0;
-#pragma unroll
-    for (int r = 0; r < S; ++r) {
-      P = -fw[l0 + r] * P;
-      d[r] = (double)d[r] + P * C;
-    }
-    // back substitution x_l = bd_l y_l - bu_l x_{l+1}, segment-local
-    double x = 0.0, Q = 1.0;
-#pragma unroll
-    for (int r = S - 1; r >= 0; --r) {
-      const double u = bu[l0 + r];
-      x = bd[l0 + r] * (double)d[r] - u * x;
-      Q = -u * Q;
-      d[r] = x;
-    }
-    bx[seg * L + ll] = x;
-    bQ[seg * L + ll] = Q;
-    __syncthreads();
-    double X = 0.0;
-    for (int g = G - 1; g > seg; --g) X = bx[g * L + ll] + bQ[g * L + ll] * X;
-    Q = 1.0;
-#pragma unroll
-    for (int r = S - 1; r >= 0; --r) {
-      Q = -bu[l0 + r] * Q;
-      d[r] = (double)d[r] + Q * X;
-    }
-  }
+template <int MODE> struct ModeTraits {
+  static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY;
+  static constexpr bool impl = MODE == LINE_IMPLICIT || MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_YZY;
+  static constexpr bool expl = MODE != LINE_IMPLICIT;
+};
 
-  if (MODE != LINE_IMPLICIT) {
-    // explicit half-step of the same direction: u + c (ea (u+ - u) - eb (u - u-))
-    xf[seg * L + ll] = (double)d[0];
-    xl[seg * L + ll] = (double)d[S - 1];
-    __syncthreads();
-    const TS cc = (TS)c;
-    TS prev = (seg > 0) ? (TS)xl[(seg - 1) * L + ll] : (TS)d[0];
-    const TS edge = (l0 + S < N) ? (TS)xf[(seg + 1) * L + ll] : (TS)d[S - 1];
-#pragma unroll
-    for (int r = 0; r < S; ++r) {
-      const TS x = (TS)d[r];
-      const TS xp = (r < S - 1) ? (TS)d[r + 1] : edge;
-      const TS o = x + cc * ((TS)ea[l0 + r] * (xp - x) - (TS)eb[l0 + r] * (x - prev));
-      prev = x;
-      d[r] = (RT)o;
-    }
-  }
+template <class Cfg>
+constexpr size_t line_lds_doubles(bool halo) {
+  return 2 * Cfg::NW * 64 + kSlots * 3 * Cfg::NMAX + (halo ? kZcMax * 128 + (Cfg::NMAX + 1) / 2 : 0);
+}
+
+GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous range of tiles so neighbouring k-windows of a line share an L2.
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = b % 8, i = b / 8;
+  return x * per + (x < rem ? x : rem) + i;
 }
 
 template <class Cfg, class TS, int MODE>
 __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* out, const TS* in,
-                                                       double c) {
-  using RT = typename std::conditional<MODE == LINE_EXPLICIT, TS, double>::type;
-  constexpr int S = Cfg::S, L = Cfg::L, G = Cfg::G, NM = Cfg::NMAX;
+                                                       double c, double a1, double a2, double a3) {
+  using MT = ModeTraits<MODE>;
+  using RT = typename std::conditional<MT::impl, double, TS>::type;
+  constexpr int S = Cfg::S, NW = Cfg::NW, NM = Cfg::NMAX;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* sh_carry = reinterpret_cast<double*>(smem_raw);  // 6*G*L doubles
-  double* sh_coef = sh_carry + 6 * G * L;                  // 5*NMAX doubles
+  double* shA = reinterpret_cast<double*>(smem_raw);  // [NW][64] carries / edges
+  double* shB = shA + NW * 64;                         // [NW][64]
+  double* shC = shB + NW * 64;                         // [kSlots][3][NM]: bd, ea, eb
+  double* shZ = shC + kSlots * 3 * NM;                 // [kZcMax][64][2]: z coefficients
+  int* shK = reinterpret_cast<int*>(shZ + kZcMax * 128);  // [NM] column class per line element
 
-  const LineTile T = (dir == 0 ? v.xtiles : v.ytiles)[blockIdx.x];
+  const LineTile* tiles = MT::halo ? v.htiles : (dir == 0 ? v.xtiles : v.ytiles);
+  const int64_t ntiles = MT::halo ? v.htiles_total : (dir == 0 ? v.xtiles_total : v.ytiles_total);
+  const LineTile T = tiles[xcd_remap(blockIdx.x, ntiles)];
   const ScenDev S_ = v.sc[T.scen];
   const int tid = threadIdx.x;
-  const int ll = tid % L;
-  const int seg = (L == 64) ? __builtin_amdgcn_readfirstlane(tid / 64) : tid / L;
+  const int lane = tid & 63;
+  const int seg = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int N = dir == 0 ? S_.nx : S_.ny;
-  const int64_t stride = dir == 0 ? (int64_t)S_.nz * S_.ny : (int64_t)S_.nz;
-  // idle lanes shadow the tile's last line; the lane offset is a 32-bit
-  // unsigned value over a wave-uniform base so loads/stores use the
-  // scalar-base addressing form (no 64-bit address per element in VGPRs)
-  const unsigned kb = (unsigned)(ll < T.nk ? ll : T.nk - 1) * (unsigned)sizeof(TS);
-  const int64_t base = S_.off + (dir == 0 ? (int64_t)S_.nz * T.line : (int64_t)S_.nz * S_.ny * T.line) + T.k0;
+  const int nz = S_.nz;
+  const int k = T.kw + lane;
+  const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
+  const int kmin = T.kw < 0 ? 0 : T.kw;
+  const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(TS);
+  const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
+  const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
 
-  // stage the coefficient set, identity rows beyond N
-  const double* g = v.coef + v.set_off[T.sid];
-  for (int q = tid; q < NM; q += Cfg::THREADS) {
-    const bool ok = q < N;
-    sh_coef[q] = ok ? g[q] : 0.0;
-    sh_coef[NM + q] = ok ? g[N + q] : 1.0;
-    sh_coef[2 * NM + q] = ok ? g[2 * N + q] : 0.0;
-    sh_coef[3 * NM + q] = ok ? g[3 * N + q] : 0.0;
-    sh_coef[4 * NM + q] = ok ? g[4 * N + q] : 0.0;
+  // coefficient sets of the window's lanes, identity rows beyond N
+  const int32_t* sids = v.setid + (dir == 0 ? S_.o_xset : S_.o_yset) + (int64_t)T.line * S_.nzone;
+  const int sid = sids[v.zone[S_.o_zone + kc]];
+  const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
+  for (int q = 0; q < T.nset; ++q) {
+    const double* g = v.coef + v.set_off[T.set[q]];
+    double* dst = shC + q * 3 * NM;
+    for (int l = tid; l < NM; l += Cfg::THREADS) {
+      const bool ok = l < N;
+      dst[l] = ok ? g[l] : 1.0;
+      dst[NM + l] = ok ? g[N + l] : 0.0;
+      dst[2 * NM + l] = ok ? g[2 * N + l] : 0.0;
+    }
+  }
+  if (MT::halo) {
+    const double* zc = v.zcoef + 2 * S_.o_zcoef;
+    const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
+    for (int q = seg; q < ncls; q += NW) {
+      shZ[(q * 64 + lane) * 2] = zc[2 * ((int64_t)q * nz + kc)];
+      shZ[(q * 64 + lane) * 2 + 1] = zc[2 * ((int64_t)q * nz + kc) + 1];
+    }
+    const uint16_t* clsl = v.cls + S_.o_cls + (int64_t)S_.ny * T.line;
+    for (int l = tid; l < NM; l += Cfg::THREADS) shK[l] = clsl[l < N ? l : N - 1];
   }
 
   RT d[S];
@@ -278,35 +271,220 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   const TS* pin = in + useg;
   if (l0 + S <= N) {
 #pragma unroll
-    for (int r = 0; r < S; ++r) {
-      const TS* pr = pin + (int64_t)r * ustride;
-      d[r] = (RT)*lane_ptr(pr, kb);
-    }
+    for (int r = 0; r < S; ++r) d[r] = (RT)*lane_ptr(pin + (int64_t)r * ustride, kb);
   } else {
 #pragma unroll
-    for (int r = 0; r < S; ++r) {
-      const TS* pr = pin + (int64_t)r * ustride;
-      d[r] = (l0 + r < N) ? (RT)*lane_ptr(pr, kb) : (RT)0;
-    }
+    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? (RT)*lane_ptr(pin + (int64_t)r * ustride, kb) : (RT)0;
   }
   __syncthreads();
 
-  line_tile_body<Cfg, TS, MODE>(sh_coef, d, sh_carry, N, seg, ll, c);
+  const double* bd = shC + slot * 3 * NM;
+  const double* ea = bd + NM;
+  const double* eb = bd + 2 * NM;
 
-  if (ll < T.nk) {
+  // Every element loop below runs in chunks of KC elements: the LDS operands
+  // of the next chunk are fetched while the current one computes, and a
+  // scheduling barrier keeps the compiler from hoisting further (which would
+  // spill the register-resident line).
+  constexpr int KC = 4;
+  static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
+  if (MT::impl) {
+    // forward: y_l = b_l - w_l y_{l-1},  w_l = -c eb_l bd_{l-1}
+    auto wmul = [&](int l) { return -c * eb[l] * bd[l > 0 ? l - 1 : 0]; };
+    double y = 0.0, P = 1.0, wn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + q);
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + r0 + KC + q);
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        y = (double)d[r0 + q] - wc[q] * y;
+        P = -wc[q] * P;
+        d[r0 + q] = y;
+      }
+      GW_SCHED_FENCE();
+    }
+    shA[seg * 64 + lane] = y;
+    shB[seg * 64 + lane] = P;
+    __syncthreads();
+    double C = 0.0;
+    for (int g = 0; g < seg; ++g) C = shA[g * 64 + lane] + shB[g * 64 + lane] * C;
+    __syncthreads();
+    P = 1.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + q);
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + r0 + KC + q);
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        P = -wc[q] * P;
+        d[r0 + q] = (double)d[r0 + q] + P * C;
+      }
+      GW_SCHED_FENCE();
+    }
+    // backward: x_l = bd_l y_l - u_l x_{l+1},  u_l = -c ea_l bd_l
+    double x = 0.0, Q = 1.0, bn[KC], un[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int l = l0 + S - 1 - q;
+      bn[q] = bd[l];
+      un[q] = -c * ea[l] * bn[q];
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double bc[KC], uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { bc[q] = bn[q]; uc[q] = un[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const int l = l0 + S - 1 - (r0 + KC + q);
+          bn[q] = bd[l];
+          un[q] = -c * ea[l] * bn[q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        x = bc[q] * (double)d[r] - uc[q] * x;
+        Q = -uc[q] * Q;
+        d[r] = x;
+      }
+      GW_SCHED_FENCE();
+    }
+    shA[seg * 64 + lane] = x;
+    shB[seg * 64 + lane] = Q;
+    __syncthreads();
+    double X = 0.0;
+    for (int g = NW - 1; g > seg; --g) X = shA[g * 64 + lane] + shB[g * 64 + lane] * X;
+    __syncthreads();
+    Q = 1.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int l = l0 + S - 1 - q;
+      un[q] = -c * ea[l] * bd[l];
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) uc[q] = un[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const int l = l0 + S - 1 - (r0 + KC + q);
+          un[q] = -c * ea[l] * bd[l];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        Q = -uc[q] * Q;
+        d[r] = (double)d[r] + Q * X;
+      }
+      GW_SCHED_FENCE();
+    }
+  }
+
+  if (MT::halo) {
+    // z-step R(dt A_z) across lanes (lane = k), column class per line element;
+    // neighbours k-1 / k+1 come from lanes -1 / +1 by DPP wave shifts (the
+    // window edge lanes keep their own value: they are halo, never output)
+    const TS al[3] = {(TS)a1, (TS)a2, (TS)a3};
+    TS ean[KC], ebn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int cl = shK[l0 + q];
+      ean[q] = (TS)shZ[(cl * 64 + lane) * 2];
+      ebn[q] = (TS)shZ[(cl * 64 + lane) * 2 + 1];
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      TS eac[KC], ebc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { eac[q] = ean[q]; ebc[q] = ebn[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const int cl = shK[l0 + r0 + KC + q];
+          ean[q] = (TS)shZ[(cl * 64 + lane) * 2];
+          ebn[q] = (TS)shZ[(cl * 64 + lane) * 2 + 1];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const TS u0 = (TS)d[r0 + q];
+        TS w = u0;
+        // kept as a real loop: unrolling the stages exposes so much ILP across
+        // elements that the register-resident line spills
+#pragma unroll 1
+        for (int st = 0; st < 3; ++st) {
+          const TS wm = wave_shr1(w);
+          const TS wp = wave_shl1(w);
+          w = u0 + al[st] * (eac[q] * (wp - w) - ebc[q] * (w - wm));
+        }
+        d[r0 + q] = (RT)w;
+      }
+      GW_SCHED_FENCE();
+    }
+  }
+
+  if (MT::expl) {
+    // explicit half-step of the line direction: u + c (ea (u+ - u) - eb (u - u-))
+    shA[seg * 64 + lane] = (double)d[0];
+    shB[seg * 64 + lane] = (double)d[S - 1];
+    __syncthreads();
+    const TS cc = (TS)c;
+    TS prev = (seg > 0) ? (TS)shB[(seg - 1) * 64 + lane] : (TS)d[0];
+    const TS edge = (l0 + S < N) ? (TS)shA[(seg + 1) * 64 + lane] : (TS)d[S - 1];
+    TS an[KC], bn2[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) { an[q] = (TS)ea[l0 + q]; bn2[q] = (TS)eb[l0 + q]; }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      TS ac[KC], bc2[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { ac[q] = an[q]; bc2[q] = bn2[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) { an[q] = (TS)ea[l0 + r0 + KC + q]; bn2[q] = (TS)eb[l0 + r0 + KC + q]; }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = r0 + q;
+        const TS x = (TS)d[r];
+        const TS xp = (r < S - 1) ? (TS)d[r + 1] : edge;
+        const TS o = x + cc * (ac[q] * (xp - x) - bc2[q] * (x - prev));
+        prev = x;
+        d[r] = (RT)o;
+      }
+      GW_SCHED_FENCE();
+    }
+  }
+
+  if (k >= T.ko0 && k < T.ko1) {
     TS* pout = out + useg;
     if (l0 + S <= N) {
 #pragma unroll
-      for (int r = 0; r < S; ++r) {
-        TS* pr = pout + (int64_t)r * ustride;
-        *lane_ptr(pr, kb) = (TS)d[r];
-      }
+      for (int r = 0; r < S; ++r) *lane_ptr(pout + (int64_t)r * ustride, kb) = (TS)d[r];
     } else {
 #pragma unroll
-      for (int r = 0; r < S; ++r) {
-        TS* pr = pout + (int64_t)r * ustride;
-        if (l0 + r < N) *lane_ptr(pr, kb) = (TS)d[r];
-      }
+      for (int r = 0; r < S; ++r)
+        if (l0 + r < N) *lane_ptr(pout + (int64_t)r * ustride, kb) = (TS)d[r];
     }
   }
 }
@@ -443,45 +621,58 @@ hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double
   return zstep_t<float>(v, (float*)phi, dt, c3, st);
 }
 
+template <class Cfg, class TS, int MODE>
+static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
+                        double a3, hipStream_t st) {
+  using MT = ModeTraits<MODE>;
+  const int64_t ntiles = MT::halo ? v.htiles_total : (dir == 0 ? v.xtiles_total : v.ytiles_total);
+  if (ntiles == 0) return;
+  const size_t sh = line_lds_doubles<Cfg>(MT::halo) * sizeof(double);
+  hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
+                     c, a1, a2, a3);
+}
+
 template <class Cfg, class TS>
-static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const TS* in, double c,
-                           int nmax, hipStream_t st) {
-  const int64_t ntiles = dir == 0 ? v.xtiles_total : v.ytiles_total;
-  (void)nmax;
-  const size_t sh = (size_t)(6 * Cfg::G * Cfg::L + 5 * Cfg::NMAX) * sizeof(double);
-  const dim3 grid((unsigned)ntiles), block(Cfg::THREADS);
-  if (mode == LINE_EXPLICIT)
-    hipLaunchKernelGGL((k_line<Cfg, TS, LINE_EXPLICIT>), grid, block, sh, st, v, dir, out, in, c);
-  else if (mode == LINE_IMPLICIT)
-    hipLaunchKernelGGL((k_line<Cfg, TS, LINE_IMPLICIT>), grid, block, sh, st, v, dir, out, in, c);
-  else
-    hipLaunchKernelGGL((k_line<Cfg, TS, LINE_IMPLICIT_EXPLICIT>), grid, block, sh, st, v, dir, out, in, c);
+static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const TS* in, double c, double a1,
+                           double a2, double a3, hipStream_t st) {
+  switch (mode) {
+    case LINE_EXPLICIT: line_launch<Cfg, TS, LINE_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, st); break;
+    case LINE_IMPLICIT: line_launch<Cfg, TS, LINE_IMPLICIT>(v, dir, out, in, c, a1, a2, a3, st); break;
+    case LINE_IMPLICIT_EXPLICIT:
+      line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, st);
+      break;
+    case LINE_ZY: line_launch<Cfg, TS, LINE_ZY>(v, dir, out, in, c, a1, a2, a3, st); break;
+    case LINE_YZY: line_launch<Cfg, TS, LINE_YZY>(v, dir, out, in, c, a1, a2, a3, st); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-// Tile shapes: lines per tile 64, waves per tile NW, elements per lane S.
-using Cfg48 = LineCfg<64, 1, 48>;
-using Cfg96 = LineCfg<64, 2, 48>;
-using Cfg192 = LineCfg<64, 4, 48>;
-using Cfg416 = LineCfg<64, 8, 52>;
-using Cfg640 = LineCfg<64, 8, 80>;
+// Tile shapes: waves per tile NW, line elements per lane S.
+using Cfg48 = LineCfg<1, 48>;
+using Cfg96 = LineCfg<2, 48>;
+using Cfg192 = LineCfg<4, 48>;
+using Cfg416 = LineCfg<8, 52>;
+using Cfg640 = LineCfg<8, 80>;
 
 template <class TS>
-static hipError_t line_t(const DevView& v, int mode, int dir, TS* out, const TS* in, double c,
-                         int nmax, hipStream_t st) {
-  if (nmax <= Cfg48::NMAX) return line_cfg<Cfg48>(v, mode, dir, out, in, c, nmax, st);
-  if (nmax <= Cfg96::NMAX) return line_cfg<Cfg96>(v, mode, dir, out, in, c, nmax, st);
-  if (nmax <= Cfg192::NMAX) return line_cfg<Cfg192>(v, mode, dir, out, in, c, nmax, st);
-  if (nmax <= Cfg416::NMAX) return line_cfg<Cfg416>(v, mode, dir, out, in, c, nmax, st);
-  if (nmax <= Cfg640::NMAX) return line_cfg<Cfg640>(v, mode, dir, out, in, c, nmax, st);
+static hipError_t line_t(const DevView& v, int mode, int dir, TS* out, const TS* in, double c, double a1,
+                         double a2, double a3, int nmax, hipStream_t st) {
+  if (nmax <= Cfg48::NMAX) return line_cfg<Cfg48>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  if (nmax <= Cfg96::NMAX) return line_cfg<Cfg96>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  if (nmax <= Cfg192::NMAX) return line_cfg<Cfg192>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  if (nmax <= Cfg416::NMAX) return line_cfg<Cfg416>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  if (nmax <= Cfg640::NMAX) return line_cfg<Cfg640>(v, mode, dir, out, in, c, a1, a2, a3, st);
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_line(const DevView& v, int prec, int mode, int dir, void* out, const void* in,
-                       double c, hipStream_t st) {
+hipError_t launch_line(const DevView& v, int prec, int mode, int dir, void* out, const void* in, double c,
+                       double dt_z, double c3, hipStream_t st) {
   const int nmax = dir == 0 ? v.nx_max : v.ny_max;
-  if (prec == 0) return line_t<double>(v, mode, dir, (double*)out, (const double*)in, c, nmax, st);
-  return line_t<float>(v, mode, dir, (float*)out, (const float*)in, c, nmax, st);
+  const double a1 = 2.0 * c3 * dt_z, a2 = 0.5 * dt_z, a3 = dt_z;
+  if (prec == 0)
+    return line_t<double>(v, mode, dir, (double*)out, (const double*)in, c, a1, a2, a3, nmax, st);
+  return line_t<float>(v, mode, dir, (float*)out, (const float*)in, c, a1, a2, a3, nmax, st);
 }
 
 hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st) {

@@ -15,6 +15,7 @@ struct DevView {
   const int64_t* col_prefix;    // [nscen+1] (i, j) columns
   const LineTile* xtiles;       // x-line tiles
   const LineTile* ytiles;       // y-line tiles
+  const LineTile* htiles;       // y-line windows with a 3-lane k halo (fused z-step)
   const uint16_t* cls;
   const int32_t* zone;
   const double* zcoef;          // pairs {ea, eb}
@@ -35,15 +36,16 @@ struct DevView {
   double* inlet;                // [nbh] inlet value of the current sub-step
   const double* inlet_tab;      // [nbh] per-well table
   int32_t nx_max, ny_max, nz_max, countz_max;
-  int64_t cells_total, cols_total, xtiles_total, ytiles_total;
+  int64_t cells_total, cols_total, xtiles_total, ytiles_total, htiles_total;
 };
 
-enum LineMode { LINE_EXPLICIT = 0, LINE_IMPLICIT = 1, LINE_IMPLICIT_EXPLICIT = 2 };
+enum LineMode { LINE_EXPLICIT = 0, LINE_IMPLICIT = 1, LINE_IMPLICIT_EXPLICIT = 2, LINE_ZY = 3, LINE_YZY = 4 };
 
 hipError_t launch_zrhs(const DevView& v, int prec, void* dphi, const void* phi, hipStream_t st);
 hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double c3, hipStream_t st);
+// c: sub-step of the line operators; dt_z, c3: the fused z-step (modes ZY, YZY).
 hipError_t launch_line(const DevView& v, int prec, int mode, int dir, void* out, const void* in,
-                       double c, hipStream_t st);
+                       double c, double dt_z, double c3, hipStream_t st);
 // Outlet sums, annulus level means and inlet values of one sub-step.
 hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st);
 // Semi-Lagrangian update of every pipe column, in place.

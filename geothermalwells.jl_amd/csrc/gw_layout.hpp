@@ -66,20 +66,22 @@ struct Dep {
   double a;
 };
 
-// A line tile: lines k0 .. k0+nk-1 (nk <= 64) of x-line j (or y-line i) =
-// `line` in scenario `scen`, all using coefficient set `sid`.
+// A line tile: the 64 lanes k = kw .. kw+63 (clamped to the grid) of x-line
+// j (or y-line i) = `line` in scenario `scen`; lanes with ko0 <= k < ko1 are
+// written.  Its lanes use the nset (<= 3) coefficient sets in `set`.
 struct LineTile {
-  int32_t scen, line, k0, nk, sid, pad0, pad1, pad2;
+  int32_t scen, line, kw, ko0, ko1, nset;
+  int32_t set[3];
+  int32_t pad;
 };
 
-// Coefficient set of one 1-D line (length N), stored as 5 consecutive arrays:
-//   fw[N]  forward multipliers  y_l = b_l - fw_l y_{l-1}
-//   bd[N]  1 / pivot            x_l = bd_l y_l - bu_l x_{l+1}
-//   bu[N]  upper / pivot
+// Coefficient set of one 1-D line (length N), stored as 3 consecutive arrays:
+//   bd[N]  1 / pivot of (I - cA) = LU (Thomas without pivoting)
 //   ea[N]  explicit coefficient of (u_{l+1} - u_l)
 //   eb[N]  explicit coefficient of (u_l - u_{l-1})
-// so that (A u)_l = ea_l (u_{l+1} - u_l) - eb_l (u_l - u_{l-1}) and
-// (I - cA) = LU with the multipliers and pivots above.
-enum { SET_FW = 0, SET_BD = 1, SET_BU = 2, SET_EA = 3, SET_EB = 4, SET_NARR = 5 };
+// so that (A u)_l = ea_l (u_{l+1} - u_l) - eb_l (u_l - u_{l-1}); the LU
+// multipliers follow as fw_l = -c eb_l bd_{l-1} (forward, y_l = b_l - fw_l
+// y_{l-1}) and bu_l = -c ea_l bd_l (backward, x_l = bd_l y_l - bu_l x_{l+1}).
+enum { SET_BD = 0, SET_EA = 1, SET_EB = 2, SET_NARR = 3 };
 
 }  // namespace gw

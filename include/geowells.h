@@ -141,6 +141,14 @@ int gw_explicit(gw_ctx *ctx, void *out, const void *in, double c, int32_t dir);
 int gw_implicit(gw_ctx *ctx, void *out, const void *in, double c, int32_t dir);
 int gw_advect(gw_ctx *ctx, void *phi, double c, double t);
 
+/* One line operator of the fused step pipeline, for callers that sequence
+ * steps themselves and for testing: mode 0 explicit (I + cA_dir), 1 implicit
+ * (I - cA_dir)^-1, 2 implicit then explicit, 3 z-step R(dt_z A_z) then
+ * explicit y, 4 implicit y then z-step then explicit y (the reference's
+ * src/solver.jl:461, then :23-31 under ROCK2 and :444 of the next step).
+ * Modes 3 and 4 are y-direction only and need out != in. */
+int gw_line_op(gw_ctx *ctx, void *out, const void *in, double c, double dt_z, int32_t dir, int32_t mode);
+
 /* Introspection (cache fields the reference exposes). */
 int64_t gw_state_elems(gw_ctx *ctx);
 int64_t gw_scenario_offset(gw_ctx *ctx, int32_t scen);
@@ -154,9 +162,11 @@ int gw_sync(gw_ctx *ctx);
 void *gw_stream(gw_ctx *ctx);
 
 /* Measurement: nsteps full steps (no graph) with HIP events between the
- * operators, on the context's stream; ms[0..5] accumulate the time of
- * z-step, explicit y, advection sub-step 1, implicit+explicit x, advection
- * sub-step 2, implicit y.  Synchronises. */
+ * kernels of the fused step pipeline, on the context's stream.  ms[] adds up:
+ * [0] z-step + explicit y of the first step (once), [1] advection sub-step 1,
+ * [2] implicit x + explicit x, [3] advection sub-step 2, [4] implicit y fused
+ * with the next step's z-step and explicit y (nsteps-1 launches), [5] the
+ * last step's implicit y (once).  Synchronises. */
 int gw_profile_steps(gw_ctx *ctx, void *phi, double t, double dt, int32_t nsteps, double *ms);
 
 #ifdef __cplusplus
