@@ -64,6 +64,71 @@ def cpu_baseline(w, threads, steps):
                        f"line-per-thread Thomas, quadratic turnaround loop as in the reference)")
 
 
+def _init_state(gw, cache, ws, dtype):
+    import torch
+    u = cache.new_state()
+    for s, w in enumerate(ws):
+        gz = np.asarray(w["gridz"], dtype=np.float64)
+        col = torch.from_numpy(w["ic"]["T_surface"] + w["ic"]["gradient"] * gz)
+        cache.scenario_view(u, s).copy_(col.to(dtype=u.dtype).expand(*cache.scenario_shape(s)))
+    return u
+
+
+def _timed_steps(gw, cache, u, dt, warmup, steps):
+    import torch
+    lib = cache._lib
+    ptr = cache.check_state(u)
+    gw._abi.check(lib.gw_steps(cache.ctx, ptr, 0.0, dt, warmup))
+    cache.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gw._abi.check(lib.gw_steps(cache.ctx, ptr, warmup * dt, dt, steps))
+    cache.sync()
+    return time.perf_counter() - t0
+
+
+def bench_batched(gw, steps, warmup, err_steps):
+    """The 64 ragged single-borehole scenarios: one scenario at a time in FP64
+    (a context each) vs all 64 in one context (every launch covers all of
+    them) in FP64 and in FP32 state; the FP32 outlet error is taken against
+    the batched FP64 run of identical inputs after the same steps."""
+    import torch
+    from geothermalwells_jl_amd import workloads as W
+    ws = W.c2()
+    dt = ws[0]["dt"]
+    cells = [W.cells(w) for w in ws]
+    total = sum(cells)
+    t_plain = 0.0
+    for w in ws:
+        cache = gw.create_cache(precision="f64", **W.scenario_kwargs(w))
+        u = _init_state(gw, cache, [w], torch.float64)
+        t_plain += _timed_steps(gw, cache, u, dt, warmup, steps)
+        del cache, u
+    res = {"workload": f"{len(ws)} ragged single-borehole scenarios "
+                       f"({min(len(w['gridx']) for w in ws)}-{max(len(w['gridx']) for w in ws)} points in x and y, "
+                       f"{min(len(w['gridz']) for w in ws)}-{max(len(w['gridz']) for w in ws)} in z), dt 120 s",
+           "cells": total, "steps": steps}
+    res["plain_fp64_one_at_a_time"] = total * steps / t_plain
+    outs = {}
+    n_err = max(err_steps, warmup + steps)
+    for prec, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        cache = gw.create_batch_cache([W.scenario_kwargs(w) for w in ws], precision=prec)
+        u = _init_state(gw, cache, ws, dtype)
+        t = _timed_steps(gw, cache, u, dt, warmup, steps)
+        res[f"batched_{prec}"] = total * steps / t
+        rest = n_err - warmup - steps
+        if rest > 0:
+            gw._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), (warmup + steps) * dt, dt, rest))
+        outs[prec] = cache.T_outlet
+        del cache, u
+    res["outlet_err_after_steps"] = n_err
+    res["speedup_batched_fp32_vs_plain_fp64"] = res["batched_f32"] / res["plain_fp64_one_at_a_time"]
+    res["speedup_batched_fp64_vs_plain_fp64"] = res["batched_f64"] / res["plain_fp64_one_at_a_time"]
+    res["outlet_max_abs_err_fp32_vs_fp64_K"] = float(np.abs(outs["f32"] - outs["f64"]).max())
+    res["unit"] = "cell-steps/s"
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +138,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=30,
+                    help="timed steps of the 64-scenario batched comparison (0: skip)")
+    ap.add_argument("--batched-err-steps", type=int, default=720,
+                    help="steps after which the FP32 outlet error of the batched comparison is taken")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -163,6 +232,10 @@ def main():
         "outlet_T": [float(x) for x in T_out],
         "state_finite": state_ok,
     }
+    if args.batched_steps > 0:
+        del cache, u
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(gw, args.batched_steps, 3, args.batched_err_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -504,9 +504,11 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   for (size_t q = 0; q < ctx.sets.size(); ++q) {
     const LineSet& L = ctx.sets[q];
     const int N = L.N;
-    double* bd = &coef[set_off[q]];
-    double* ea = bd + N;
-    double* eb = bd + 2 * N;
+    double* bd = &coef[set_off[q]] + SET_BD * N;
+    double* ea = &coef[set_off[q]] + SET_EA * N;
+    double* eb = &coef[set_off[q]] + SET_EB * N;
+    double* fw = &coef[set_off[q]] + SET_FW * N;
+    double* bu = &coef[set_off[q]] + SET_BU * N;
     // (I - cA): lower_l = -c eb_l, diag_l = 1 + c (ea_l + eb_l), upper_l = -c ea_l
     double piv = 1.0 + c * (L.ea[0] + L.eb[0]);
     for (int l = 1; l < N; ++l) {
@@ -517,7 +519,12 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       piv = (1.0 + c * (L.ea[l] + L.eb[l])) - w * upper_prev;
     }
     bd[N - 1] = 1.0 / piv;
-    for (int l = 0; l < N; ++l) { ea[l] = L.ea[l]; eb[l] = L.eb[l]; }
+    for (int l = 0; l < N; ++l) {
+      ea[l] = L.ea[l];
+      eb[l] = L.eb[l];
+      fw[l] = (-c * L.eb[l]) * bd[l > 0 ? l - 1 : 0];
+      bu[l] = (-c * L.ea[l]) * bd[l];
+    }
   }
   // departure tables
   std::vector<Dep> dep;

@@ -195,11 +195,16 @@ template <int MODE> struct ModeTraits {
   static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY;
   static constexpr bool impl = MODE == LINE_IMPLICIT || MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_YZY;
   static constexpr bool expl = MODE != LINE_IMPLICIT;
+  // coefficient arrays staged per set slot: bd, fw, bu (implicit) then ea, eb
+  // (explicit), each NMAX long
+  static constexpr int NA = (impl ? 3 : 0) + (expl ? 2 : 0);
+  static constexpr int A_BD = 0, A_FW = 1, A_BU = 2, A_EA = impl ? 3 : 0, A_EB = A_EA + 1;
 };
 
-template <class Cfg>
-constexpr size_t line_lds_doubles(bool halo) {
-  return 2 * Cfg::NW * 64 + kSlots * 3 * Cfg::NMAX + (halo ? kZcMax * 128 + (Cfg::NMAX + 1) / 2 : 0);
+template <class Cfg, int MODE>
+constexpr size_t line_lds_doubles() {
+  using MT = ModeTraits<MODE>;
+  return 2 * Cfg::NW * 64 + kSlots * MT::NA * Cfg::NMAX + (MT::halo ? kZcMax * 128 + (Cfg::NMAX + 1) / 2 : 0);
 }
 
 GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
@@ -214,13 +219,15 @@ template <class Cfg, class TS, int MODE>
 __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* out, const TS* in,
                                                        double c, double a1, double a2, double a3) {
   using MT = ModeTraits<MODE>;
-  using RT = typename std::conditional<MT::impl, double, TS>::type;
+  // The line is held in registers in the storage type; every recurrence runs
+  // in FP64 and rounds to the storage type only when it writes an element.
+  using RT = TS;
   constexpr int S = Cfg::S, NW = Cfg::NW, NM = Cfg::NMAX;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* shA = reinterpret_cast<double*>(smem_raw);  // [NW][64] carries / edges
   double* shB = shA + NW * 64;                         // [NW][64]
-  double* shC = shB + NW * 64;                         // [kSlots][3][NM]: bd, ea, eb
-  double* shZ = shC + kSlots * 3 * NM;                 // [kZcMax][64][2]: z coefficients
+  double* shC = shB + NW * 64;                         // [kSlots][MT::NA][NM] coefficient arrays
+  double* shZ = shC + kSlots * MT::NA * NM;            // [kZcMax][64][2]: z coefficients
   int* shK = reinterpret_cast<int*>(shZ + kZcMax * 128);  // [NM] column class per line element
 
   const LineTile* tiles = MT::halo ? v.htiles : (dir == 0 ? v.xtiles : v.ytiles);
@@ -245,12 +252,18 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
   for (int q = 0; q < T.nset; ++q) {
     const double* g = v.coef + v.set_off[T.set[q]];
-    double* dst = shC + q * 3 * NM;
+    double* dst = shC + q * MT::NA * NM;
     for (int l = tid; l < NM; l += Cfg::THREADS) {
       const bool ok = l < N;
-      dst[l] = ok ? g[l] : 1.0;
-      dst[NM + l] = ok ? g[N + l] : 0.0;
-      dst[2 * NM + l] = ok ? g[2 * N + l] : 0.0;
+      if (MT::impl) {
+        dst[MT::A_BD * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
+        dst[MT::A_FW * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
+        dst[MT::A_BU * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
+      }
+      if (MT::expl) {
+        dst[MT::A_EA * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
+        dst[MT::A_EB * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
+      }
     }
   }
   if (MT::halo) {
@@ -278,9 +291,12 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   }
   __syncthreads();
 
-  const double* bd = shC + slot * 3 * NM;
-  const double* ea = bd + NM;
-  const double* eb = bd + 2 * NM;
+  const double* cs = shC + slot * MT::NA * NM;
+  const double* bd = cs + MT::A_BD * NM;
+  const double* fw = cs + MT::A_FW * NM;
+  const double* bu = cs + MT::A_BU * NM;
+  const double* ea = cs + MT::A_EA * NM;
+  const double* eb = cs + MT::A_EB * NM;
 
   // Every element loop below runs in chunks of KC elements: the LDS operands
   // of the next chunk are fetched while the current one computes, and a
@@ -289,8 +305,8 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   constexpr int KC = 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
   if (MT::impl) {
-    // forward: y_l = b_l - w_l y_{l-1},  w_l = -c eb_l bd_{l-1}
-    auto wmul = [&](int l) { return -c * eb[l] * bd[l > 0 ? l - 1 : 0]; };
+    // forward: y_l = b_l - fw_l y_{l-1}
+    auto wmul = [&](int l) { return fw[l]; };
     double y = 0.0, P = 1.0, wn[KC];
 #pragma unroll
     for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + q);
@@ -336,13 +352,13 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
       }
       GW_SCHED_FENCE();
     }
-    // backward: x_l = bd_l y_l - u_l x_{l+1},  u_l = -c ea_l bd_l
+    // backward: x_l = bd_l y_l - bu_l x_{l+1}
     double x = 0.0, Q = 1.0, bn[KC], un[KC];
 #pragma unroll
     for (int q = 0; q < KC; ++q) {
       const int l = l0 + S - 1 - q;
       bn[q] = bd[l];
-      un[q] = -c * ea[l] * bn[q];
+      un[q] = bu[l];
     }
 #pragma unroll
     for (int r0 = 0; r0 < S; r0 += KC) {
@@ -354,7 +370,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
         for (int q = 0; q < KC; ++q) {
           const int l = l0 + S - 1 - (r0 + KC + q);
           bn[q] = bd[l];
-          un[q] = -c * ea[l] * bn[q];
+          un[q] = bu[l];
         }
       }
 #pragma unroll
@@ -374,10 +390,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
     __syncthreads();
     Q = 1.0;
 #pragma unroll
-    for (int q = 0; q < KC; ++q) {
-      const int l = l0 + S - 1 - q;
-      un[q] = -c * ea[l] * bd[l];
-    }
+    for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - q];
 #pragma unroll
     for (int r0 = 0; r0 < S; r0 += KC) {
       double uc[KC];
@@ -385,10 +398,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
       for (int q = 0; q < KC; ++q) uc[q] = un[q];
       if (r0 + KC < S) {
 #pragma unroll
-        for (int q = 0; q < KC; ++q) {
-          const int l = l0 + S - 1 - (r0 + KC + q);
-          un[q] = -c * ea[l] * bd[l];
-        }
+        for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - (r0 + KC + q)];
       }
 #pragma unroll
       for (int q = 0; q < KC; ++q) {
@@ -627,7 +637,7 @@ static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double
   using MT = ModeTraits<MODE>;
   const int64_t ntiles = MT::halo ? v.htiles_total : (dir == 0 ? v.xtiles_total : v.ytiles_total);
   if (ntiles == 0) return;
-  const size_t sh = line_lds_doubles<Cfg>(MT::halo) * sizeof(double);
+  const size_t sh = line_lds_doubles<Cfg, MODE>() * sizeof(double);
   hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
                      c, a1, a2, a3);
 }

@@ -75,13 +75,14 @@ struct LineTile {
   int32_t pad;
 };
 
-// Coefficient set of one 1-D line (length N), stored as 3 consecutive arrays:
+// Coefficient set of one 1-D line (length N) for one half-step length c,
+// stored as 5 consecutive arrays:
 //   bd[N]  1 / pivot of (I - cA) = LU (Thomas without pivoting)
 //   ea[N]  explicit coefficient of (u_{l+1} - u_l)
 //   eb[N]  explicit coefficient of (u_l - u_{l-1})
-// so that (A u)_l = ea_l (u_{l+1} - u_l) - eb_l (u_l - u_{l-1}); the LU
-// multipliers follow as fw_l = -c eb_l bd_{l-1} (forward, y_l = b_l - fw_l
-// y_{l-1}) and bu_l = -c ea_l bd_l (backward, x_l = bd_l y_l - bu_l x_{l+1}).
-enum { SET_BD = 0, SET_EA = 1, SET_EB = 2, SET_NARR = 3 };
+//   fw[N]  forward multiplier  (-c eb_l) bd_{l-1}   (y_l = b_l - fw_l y_{l-1})
+//   bu[N]  backward multiplier (-c ea_l) bd_l       (x_l = bd_l y_l - bu_l x_{l+1})
+// with (A u)_l = ea_l (u_{l+1} - u_l) - eb_l (u_l - u_{l-1}); fw_0 uses bd_0.
+enum { SET_BD = 0, SET_EA = 1, SET_EB = 2, SET_FW = 3, SET_BU = 4, SET_NARR = 5 };
 
 }  // namespace gw
