@@ -90,7 +90,8 @@ class Cache:
     have a host meaning; `T_outlet` reads the device outlet of the last
     advection sub-step."""
 
-    def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3):
+    def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
+                 line_window=None):
         self._scen = scenarios
         self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
         if self.backend.type != "cuda":
@@ -107,7 +108,9 @@ class Cache:
         prob.precision = _abi.GW_F64 if precision == "f64" else _abi.GW_F32_STATE
         prob.stream = stream
         prob.rock2_c3 = self.rock2_c3
-        prob.flags = 0
+        if line_window not in (None, 64, 32):
+            raise ValueError("line_window is None (autotune), 64 or 32")
+        prob.flags = {None: 0, 64: _abi.GW_FLAG_LINE_W64, 32: _abi.GW_FLAG_LINE_W32}[line_window]
         h = ctypes.c_void_p()
         _abi.check(lib.gw_create(ctypes.byref(prob), ctypes.byref(h)))
         self._ctx = h
@@ -199,19 +202,23 @@ class Cache:
 
 
 def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inlet_model, precision="f64",
-                 stream=None, rock2_c3=DEFAULT_ROCK2_C3):
-    """create_cache (src/cache.jl:55-116)."""
+                 stream=None, rock2_c3=DEFAULT_ROCK2_C3, line_window=None):
+    """create_cache (src/cache.jl:55-116).  `line_window` forces the k-window
+    width (64 or 32 lanes) of the line kernels; None lets the first multi-step
+    call time both and keep the faster."""
     return Cache([_Scenario(gridx, gridy, gridz, materials, boreholes, inlet_model)], backend=backend,
-                 precision=precision, stream=stream, rock2_c3=rock2_c3)
+                 precision=precision, stream=stream, rock2_c3=rock2_c3, line_window=line_window)
 
 
-def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3):
+def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
+                       line_window=None):
     """Many independent scenarios (each a dict with the create_cache keywords
     gridx, gridy, gridz, materials, boreholes, inlet_model) advanced together:
     every kernel launch covers all of them."""
     sc = [_Scenario(d["gridx"], d["gridy"], d["gridz"], d["materials"], d["boreholes"], d["inlet_model"])
           for d in scenarios]
-    return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3)
+    return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3,
+                 line_window=line_window)
 
 
 # ---------------------------------------------------------------------------

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -223,7 +224,7 @@ struct gw_ctx {
   std::vector<void*> allocs;
   ScenDev* d_sc = nullptr;
   int64_t *d_cell_prefix = nullptr, *d_col_prefix = nullptr;
-  LineTile *d_xtiles = nullptr, *d_ytiles = nullptr, *d_htiles = nullptr;
+  LineTile *d_xtiles[2] = {}, *d_ytiles[2] = {}, *d_htiles[2] = {};  // per k-window family
   uint16_t* d_cls = nullptr;
   int32_t* d_zone = nullptr;
   double* d_zcoef = nullptr;
@@ -231,7 +232,15 @@ struct gw_ctx {
   PipeCol* d_pcol = nullptr;
   double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
   int nx_max = 0, ny_max = 0, nz_max = 0, countz_max = 0;
-  int64_t cells_total = 0, cols_total = 0, xtiles_total = 0, ytiles_total = 0, htiles_total = 0;
+  int64_t cells_total = 0, cols_total = 0;
+  int64_t xtiles_total[2] = {}, ytiles_total[2] = {}, htiles_total[2] = {};
+  // k-window width (64 or 32) of each line operator (mode, direction): the
+  // length default until the first gw_steps times both widths (autotune), or
+  // the width the problem flags force
+  int lw_sel[5][2] = {};
+  int force_lw = 0;
+  bool tuned = false;
+  int lw(int mode, int dir) const { return lw_sel[mode][dir]; }
   bool fused_ok = false;  // fused z-step line kernels usable (few enough column classes)
   std::map<double, CTab> ctabs;
   bool host_inlet = false;
@@ -265,7 +274,10 @@ struct gw_ctx {
     DevView v{};
     v.sc = d_sc; v.nscen = (int)sc.size();
     v.cell_prefix = d_cell_prefix; v.col_prefix = d_col_prefix;
-    v.xtiles = d_xtiles; v.ytiles = d_ytiles; v.htiles = d_htiles;
+    for (int f = 0; f < 2; ++f) {
+      v.xtiles[f] = d_xtiles[f]; v.ytiles[f] = d_ytiles[f]; v.htiles[f] = d_htiles[f];
+      v.xtiles_total[f] = xtiles_total[f]; v.ytiles_total[f] = ytiles_total[f]; v.htiles_total[f] = htiles_total[f];
+    }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
     if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial; }
@@ -273,14 +285,11 @@ struct gw_ctx {
     v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
     v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
     v.cells_total = cells_total; v.cols_total = cols_total;
-    v.xtiles_total = xtiles_total; v.ytiles_total = ytiles_total; v.htiles_total = htiles_total;
     return v;
   }
 };
 
 namespace {
-
-constexpr int kTileL = 64;
 
 // Classification, zones and z coefficients of one scenario.
 void classify(Scen& s) {
@@ -409,16 +418,17 @@ void build_line_sets(gw_ctx& ctx, Scen& s, std::unordered_map<std::string, int>&
   }
 }
 
-// Line tiles of one scenario and direction.  Each tile is a 64-lane k window
-// whose lanes meet at most kMaxSets coefficient sets.  Plain tiles output all
-// their lanes; halo tiles (halo = 3, for the fused z-step) output lanes
-// halo .. 63-halo, the others only feed the z-stencil of the outputs.
+// Line tiles of one scenario and direction.  Each tile is a k window of lw
+// lanes (64 or 32) whose lanes meet at most kMaxSets coefficient sets.  Plain
+// tiles output all their lanes; halo tiles (halo = 3, for the fused z-step)
+// output lanes halo .. lw-1-halo, the others only feed the z-stencil of the
+// outputs.
 constexpr int kMaxSets = 3;
 
-void line_tiles(const Scen& s, int si, int dir, int halo, std::vector<LineTile>& out) {
+void line_tiles(const Scen& s, int si, int dir, int halo, int lw, std::vector<LineTile>& out) {
   const int nlines = dir == 0 ? s.ny : s.nx;
   const std::vector<int32_t>& ids = dir == 0 ? s.xset : s.yset;
-  const int width = kTileL - 2 * halo;
+  const int width = lw - 2 * halo;
   for (int line = 0; line < nlines; ++line) {
     auto sid_at = [&](int k) {
       const int kc = std::min(std::max(k, 0), s.nz - 1);
@@ -695,17 +705,17 @@ int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
   if (rc) return rc;
   DevView v = ctx.view(tab);
   // explicit y  (src/solver.jl:444)
-  rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, phi, phi, c, 0.0, 0.0, ctx.stream), "explicit y");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "explicit y");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, phi, t);  // :447
   if (rc) return rc;
   // implicit x then explicit x  (:450, :455)
-  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, phi, phi, c, 0.0, 0.0, ctx.stream), "implicit+explicit x");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), phi, phi, c, 0.0, 0.0, ctx.stream), "implicit+explicit x");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, phi, t + dt / 2);  // :458
   if (rc) return rc;
   // implicit y  (:461)
-  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, phi, phi, c, 0.0, 0.0, ctx.stream), "implicit y");
+  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "implicit y");
 }
 
 // The z-step applies R(dt A_z) = 1 + z + z^2/2 + c3 z^3 at every ROCK2 degree:
@@ -730,8 +740,12 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
   if (!p || !out) return fail(GW_EINVAL, "null argument");
   if (p->n_scen <= 0 || !p->scen) return fail(GW_EINVAL, "no scenarios");
   if (p->precision != GW_F64 && p->precision != GW_F32_STATE) return fail(GW_EINVAL, "unknown precision");
+  if ((p->flags & ~(GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32)) ||
+      (p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32)) == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32))
+    return fail(GW_EINVAL, "unknown or conflicting flags");
   std::unique_ptr<gw_ctx> ctx(new gw_ctx());
   ctx->prec = p->precision;
+  ctx->force_lw = (p->flags & GW_FLAG_LINE_W64) ? 64 : ((p->flags & GW_FLAG_LINE_W32) ? 32 : 0);
   ctx->c3 = p->rock2_c3 > 0 ? p->rock2_c3 : 1.0 / 16.0;
   ctx->zstab = stability_bound(ctx->c3);
   std::unordered_map<std::string, int> smap;
@@ -809,9 +823,20 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       hip_check(hipStreamCreate(&ctx->stream));
       ctx->own_stream = true;
     }
+    for (auto& s : ctx->sc) {
+      ctx->nx_max = std::max(ctx->nx_max, s.nx);
+      ctx->ny_max = std::max(ctx->ny_max, s.ny);
+      ctx->nz_max = std::max(ctx->nz_max, s.nz);
+      ctx->countz_max = std::max(ctx->countz_max, s.countz);
+    }
+    for (int mode = 0; mode < 5; ++mode)
+      for (int dir = 0; dir < 2; ++dir) {
+        const int n = dir == 0 ? ctx->nx_max : ctx->ny_max;
+        ctx->lw_sel[mode][dir] = ctx->force_lw ? ctx->force_lw : line_window_default(n);
+      }
     std::vector<ScenDev> sd;
     std::vector<int64_t> cellp{0}, colp{0};
-    std::vector<LineTile> xt, yt, ht;
+    std::vector<LineTile> xt[2], yt[2], ht[2];
     std::vector<uint16_t> cls;
     std::vector<int32_t> zone, setid;
     std::vector<double> zcoef, inlet_tab;
@@ -830,28 +855,33 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       const int64_t ncell = (int64_t)s.nx * s.ny * s.nz;
       cellp.push_back(cellp.back() + ncell);
       colp.push_back(colp.back() + (int64_t)s.nx * s.ny);
-      line_tiles(s, (int)sd.size() - 1, 0, 0, xt);
-      line_tiles(s, (int)sd.size() - 1, 1, 0, yt);
-      line_tiles(s, (int)sd.size() - 1, 1, 3, ht);
+      for (int f = 0; f < 2; ++f) {
+        const int lw = f == 0 ? 64 : 32;
+        if (line_window_ok(ctx->nx_max, lw)) line_tiles(s, (int)sd.size() - 1, 0, 0, lw, xt[f]);
+        if (line_window_ok(ctx->ny_max, lw)) {
+          line_tiles(s, (int)sd.size() - 1, 1, 0, lw, yt[f]);
+          line_tiles(s, (int)sd.size() - 1, 1, 3, lw, ht[f]);
+        }
+      }
       for (size_t b = 0; b < s.bh.size(); ++b)
         inlet_tab.push_back(s.inlet.kind == GW_INLET_PER_WELL ? s.per_well[b] : 0.0);
       gbh += (int)s.bh.size();
-      ctx->nx_max = std::max(ctx->nx_max, s.nx);
-      ctx->ny_max = std::max(ctx->ny_max, s.ny);
-      ctx->nz_max = std::max(ctx->nz_max, s.nz);
-      ctx->countz_max = std::max(ctx->countz_max, s.countz);
     }
     ctx->cells_total = cellp.back();
     ctx->cols_total = colp.back();
-    ctx->xtiles_total = (int64_t)xt.size();
-    ctx->ytiles_total = (int64_t)yt.size();
-    ctx->htiles_total = (int64_t)ht.size();
+    for (int f = 0; f < 2; ++f) {
+      ctx->xtiles_total[f] = (int64_t)xt[f].size();
+      ctx->ytiles_total[f] = (int64_t)yt[f].size();
+      ctx->htiles_total[f] = (int64_t)ht[f].size();
+    }
     ctx->d_sc = ctx->up(sd);
     ctx->d_cell_prefix = ctx->up(cellp);
     ctx->d_col_prefix = ctx->up(colp);
-    ctx->d_xtiles = ctx->up(xt);
-    ctx->d_ytiles = ctx->up(yt);
-    ctx->d_htiles = ctx->up(ht);
+    for (int f = 0; f < 2; ++f) {
+      ctx->d_xtiles[f] = ctx->up(xt[f]);
+      ctx->d_ytiles[f] = ctx->up(yt[f]);
+      ctx->d_htiles[f] = ctx->up(ht[f]);
+    }
     ctx->fused_ok = true;
     for (auto& s : ctx->sc) ctx->fused_ok = ctx->fused_ok && s.ncls <= 16;
     ctx->d_cls = ctx->up(cls);
@@ -936,7 +966,7 @@ int ensure_buf2(gw_ctx& ctx) {
 }
 
 int fused_zy(gw_ctx& ctx, const CTab* tab, const void* in, void* out, double dt) {
-  return hip_status(launch_line(ctx.view(tab), ctx.prec, LINE_ZY, 1, out, in, dt / 2, dt, ctx.c3, ctx.stream),
+  return hip_status(launch_line(ctx.view(tab), ctx.prec, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), out, in, dt / 2, dt, ctx.c3, ctx.stream),
                     "z-step + explicit y");
 }
 
@@ -947,22 +977,64 @@ int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end
   DevView v = ctx.view(tab);
   int rc = sub_step_advect(ctx, tab, cur, t_end);
   if (rc) return rc;
-  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, cur, cur, c, 0.0, 0.0, ctx.stream), "x");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx.stream), "x");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, cur, t_end + c);
   if (rc) return rc;
   if (last)
-    return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, next, cur, c, 0.0, 0.0, ctx.stream), "implicit y");
-  return hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, next, cur, c, dt, ctx.c3, ctx.stream),
+    return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream), "implicit y");
+  return hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, ctx.lw(LINE_YZY, 1), next, cur, c, dt, ctx.c3, ctx.stream),
                     "implicit y + z-step + explicit y");
 }
 
-int prepare_steps(gw_ctx& ctx, double dt, CTab*& tab) {
+// Line-tile autotune: each line operator of the fused pipeline is timed in
+// both k-window widths, out of place from the caller's field into the second
+// buffer (the field is only read), and the faster width is kept for the
+// context.  Runs once, before the first step graph is captured.
+int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
+  static const int ops[4][2] = {{LINE_ZY, 1}, {LINE_IMPLICIT_EXPLICIT, 0}, {LINE_YZY, 1}, {LINE_IMPLICIT, 1}};
+  const DevView v = ctx.view(tab);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_status(hipEventCreate(&e0), "event");
+  if (!rc) rc = hip_status(hipEventCreate(&e1), "event");
+  for (int q = 0; q < 4 && !rc; ++q) {
+    const int mode = ops[q][0], dir = ops[q][1];
+    const int n = dir == 0 ? ctx.nx_max : ctx.ny_max;
+    float best = 0.f;
+    int best_lw = ctx.lw_sel[mode][dir];
+    for (int lw : {64, 32}) {
+      if (!line_window_ok(n, lw)) continue;
+      auto run = [&]() {
+        return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream),
+                          "line tune");
+      };
+      if ((rc = run())) break;  // first launch: code object load
+      if ((rc = hip_status(hipEventRecord(e0, ctx.stream), "event"))) break;
+      for (int r = 0; r < 2 && !rc; ++r) rc = run();
+      if (rc || (rc = hip_status(hipEventRecord(e1, ctx.stream), "event"))) break;
+      if ((rc = hip_status(hipEventSynchronize(e1), "event sync"))) break;
+      float ms = 0.f;
+      if ((rc = hip_status(hipEventElapsedTime(&ms, e0, e1), "event time"))) break;
+      if (best == 0.f || ms < best) { best = ms; best_lw = lw; }
+    }
+    ctx.lw_sel[mode][dir] = best_lw;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int prepare_steps(gw_ctx& ctx, const void* phi, double dt, CTab*& tab) {
   int rc = build_ctab(ctx, dt / 2, tab);
   if (rc) return rc;
   rc = check_zstep(ctx, dt);
   if (rc) return rc;
-  return ensure_buf2(ctx);
+  if ((rc = ensure_buf2(ctx))) return rc;
+  if (!ctx.tuned && !ctx.force_lw) {
+    if ((rc = tune_lines(ctx, tab, phi, dt))) return rc;
+    ctx.tuned = true;
+  }
+  return GW_OK;
 }
 
 int capture_body(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t, double dt, hipGraphExec_t* out) {
@@ -1002,7 +1074,7 @@ int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
     return GW_OK;
   }
   CTab* tab = nullptr;
-  int rc = prepare_steps(*ctx, dt, tab);
+  int rc = prepare_steps(*ctx, phi, dt, tab);
   if (rc) return rc;
   void* B = ctx->buf2;
   rc = fused_zy(*ctx, tab, phi, B, dt);
@@ -1043,7 +1115,7 @@ int gw_line_op(gw_ctx* ctx, void* out, const void* in, double c, double dt_z, in
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, out, in, c, dt_z, ctx->c3, ctx->stream),
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, ctx->lw(mode, dir), out, in, c, dt_z, ctx->c3, ctx->stream),
                     "line operator");
 }
 
@@ -1080,7 +1152,7 @@ int gw_explicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, out, in, c, 0.0, 0.0, ctx->stream), "explicit");
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, ctx->lw(LINE_EXPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream), "explicit");
 }
 
 int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
@@ -1088,7 +1160,7 @@ int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, out, in, c, 0.0, 0.0, ctx->stream), "implicit");
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, ctx->lw(LINE_IMPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream), "implicit");
 }
 
 int gw_advect(gw_ctx* ctx, void* phi, double c, double t) {
@@ -1145,7 +1217,7 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
   if (!ctx->fused_ok) return fail(GW_ESTATE, "profiling follows the fused step pipeline, unavailable here");
   if (nsteps == 0) return GW_OK;
   CTab* tab = nullptr;
-  int rc = prepare_steps(*ctx, dt, tab);
+  int rc = prepare_steps(*ctx, phi, dt, tab);
   if (rc) return rc;
   hipEvent_t ev[5] = {};
   for (auto& e : ev)
@@ -1169,12 +1241,12 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
       void* cur = (n & 1) ? phi : B;
       void* next = last ? phi : ((n & 1) ? B : phi);
       if ((rc = rec(0)) || (rc = sub_step_advect(*ctx, tab, cur, te)) || (rc = rec(1))) break;
-      if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, cur, cur, c, 0.0, 0.0, ctx->stream),
+      if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, ctx->lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx->stream),
                            "x")) || (rc = rec(2)))
         break;
       if ((rc = sub_step_advect(*ctx, tab, cur, te + c)) || (rc = rec(3))) break;
-      rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, next, cur, c, 0.0, 0.0, ctx->stream), "y")
-                : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, next, cur, c, dt, ctx->c3, ctx->stream), "yzy");
+      rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, ctx->lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx->stream), "y")
+                : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, ctx->lw(LINE_YZY, 1), next, cur, c, dt, ctx->c3, ctx->stream), "yzy");
       if (rc || (rc = rec(4))) break;
       if ((rc = hip_status(hipEventSynchronize(ev[4]), "event sync"))) break;
       if ((rc = acc(0, 1, 1)) || (rc = acc(1, 2, 2)) || (rc = acc(2, 3, 3)) || (rc = acc(3, 4, last ? 5 : 4))) break;

@@ -181,11 +181,15 @@ __global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, 
 // three times across lanes (shuffles), so only lanes 3..60 of a window are
 // outputs; the halo lanes re-solve their own lines.
 // ---------------------------------------------------------------------------
-template <int NW_, int S_>
+template <int NW_, int S_, int LW_>
 struct LineCfg {
   static constexpr int NW = NW_, S = S_;
-  static constexpr int NMAX = NW_ * S_;     // longest line
+  static constexpr int LW = LW_;             // k-window width: lanes per line element
+  static constexpr int H = 64 / LW_;         // line segments per wave (one per half-wave when LW = 32)
+  static constexpr int G = NW_ * H;          // line segments per tile
+  static constexpr int NMAX = G * S_;        // longest line
   static constexpr int THREADS = NW_ * 64;
+  static_assert(LW_ == 64 || LW_ == 32, "k-window of a full or half wave");
 };
 
 constexpr int kSlots = 3;      // coefficient sets per window
@@ -204,7 +208,8 @@ template <int MODE> struct ModeTraits {
 template <class Cfg, int MODE>
 constexpr size_t line_lds_doubles() {
   using MT = ModeTraits<MODE>;
-  return 2 * Cfg::NW * 64 + kSlots * MT::NA * Cfg::NMAX + (MT::halo ? kZcMax * 128 + (Cfg::NMAX + 1) / 2 : 0);
+  return 2 * Cfg::G * Cfg::LW + kSlots * MT::NA * Cfg::NMAX +
+         (MT::halo ? kZcMax * Cfg::LW * 2 + (Cfg::NMAX + 1) / 2 : 0);
 }
 
 GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
@@ -216,33 +221,42 @@ GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
 }
 
 template <class Cfg, class TS, int MODE>
-__global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* out, const TS* in,
+__global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS* out, const TS* in,
                                                        double c, double a1, double a2, double a3) {
   using MT = ModeTraits<MODE>;
   // The line is held in registers in the storage type; every recurrence runs
   // in FP64 and rounds to the storage type only when it writes an element.
   using RT = TS;
-  constexpr int S = Cfg::S, NW = Cfg::NW, NM = Cfg::NMAX;
+  constexpr int S = Cfg::S, NM = Cfg::NMAX, LW = Cfg::LW, H = Cfg::H, G = Cfg::G;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* shA = reinterpret_cast<double*>(smem_raw);  // [NW][64] carries / edges
-  double* shB = shA + NW * 64;                         // [NW][64]
-  double* shC = shB + NW * 64;                         // [kSlots][MT::NA][NM] coefficient arrays
-  double* shZ = shC + kSlots * MT::NA * NM;            // [kZcMax][64][2]: z coefficients
-  int* shK = reinterpret_cast<int*>(shZ + kZcMax * 128);  // [NM] column class per line element
+  double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW] carries / edges
+  double* shB = shA + G * LW;                          // [G][LW]
+  double* shC = shB + G * LW;                          // [kSlots][MT::NA][NM] coefficient arrays
+  double* shZ = shC + kSlots * MT::NA * NM;            // [kZcMax][LW][2]: z coefficients
+  int* shK = reinterpret_cast<int*>(shZ + kZcMax * LW * 2);  // [NM] column class per line element
 
-  const LineTile* tiles = MT::halo ? v.htiles : (dir == 0 ? v.xtiles : v.ytiles);
-  const int64_t ntiles = MT::halo ? v.htiles_total : (dir == 0 ? v.xtiles_total : v.ytiles_total);
+  constexpr int FAM = LW == 32 ? 1 : 0;
+  const LineTile* tiles = MT::halo ? v.htiles[FAM] : (dir == 0 ? v.xtiles[FAM] : v.ytiles[FAM]);
+  const int64_t ntiles =
+      MT::halo ? v.htiles_total[FAM] : (dir == 0 ? v.xtiles_total[FAM] : v.ytiles_total[FAM]);
   const LineTile T = tiles[xcd_remap(blockIdx.x, ntiles)];
   const ScenDev S_ = v.sc[T.scen];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int seg = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int kk = lane % LW;    // k offset in the window
+  const int hh = lane / LW;    // half-wave (0 when LW = 64)
+  const int seg = wv * H + hh;
   const int N = dir == 0 ? S_.nx : S_.ny;
   const int nz = S_.nz;
-  const int k = T.kw + lane;
+  const int k = T.kw + kk;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
   const int kmin = T.kw < 0 ? 0 : T.kw;
-  const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(TS);
+  // lane byte offset from the wave's first segment: the k offset plus, for
+  // the upper half-wave, S line elements (below 2^32: nx, ny <= 640,
+  // nz <= 4096, S <= 52)
+  const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(TS) +
+                      (unsigned)(hh * S) * (unsigned)(dir == 0 ? nz * S_.ny : nz) * (unsigned)sizeof(TS);
   const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
   const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
 
@@ -251,7 +265,8 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   const int sid = sids[v.zone[S_.o_zone + kc]];
   const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
   for (int q = 0; q < T.nset; ++q) {
-    const double* g = v.coef + v.set_off[T.set[q]];
+    const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : T.set[2]);
+    const double* g = v.coef + v.set_off[sq];
     double* dst = shC + q * MT::NA * NM;
     for (int l = tid; l < NM; l += Cfg::THREADS) {
       const bool ok = l < N;
@@ -269,9 +284,9 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   if (MT::halo) {
     const double* zc = v.zcoef + 2 * S_.o_zcoef;
     const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
-    for (int q = seg; q < ncls; q += NW) {
-      shZ[(q * 64 + lane) * 2] = zc[2 * ((int64_t)q * nz + kc)];
-      shZ[(q * 64 + lane) * 2 + 1] = zc[2 * ((int64_t)q * nz + kc) + 1];
+    for (int q = wv; q < ncls; q += Cfg::NW) {
+      shZ[(q * LW + kk) * 2 + hh] = zc[2 * ((int64_t)q * nz + kc) + hh];
+      if (H == 1) shZ[(q * LW + kk) * 2 + 1] = zc[2 * ((int64_t)q * nz + kc) + 1];
     }
     const uint16_t* clsl = v.cls + S_.o_cls + (int64_t)S_.ny * T.line;
     for (int l = tid; l < NM; l += Cfg::THREADS) shK[l] = clsl[l < N ? l : N - 1];
@@ -280,7 +295,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
   RT d[S];
   const int l0 = seg * S;
   const int64_t ustride = uniform64(stride);
-  const int64_t useg = uniform64(base + (int64_t)l0 * stride);
+  const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
   const TS* pin = in + useg;
   if (l0 + S <= N) {
 #pragma unroll
@@ -327,11 +342,11 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
       }
       GW_SCHED_FENCE();
     }
-    shA[seg * 64 + lane] = y;
-    shB[seg * 64 + lane] = P;
+    shA[seg * LW + kk] = y;
+    shB[seg * LW + kk] = P;
     __syncthreads();
     double C = 0.0;
-    for (int g = 0; g < seg; ++g) C = shA[g * 64 + lane] + shB[g * 64 + lane] * C;
+    for (int g = 0; g < seg; ++g) C = shA[g * LW + kk] + shB[g * LW + kk] * C;
     __syncthreads();
     P = 1.0;
 #pragma unroll
@@ -382,11 +397,11 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
       }
       GW_SCHED_FENCE();
     }
-    shA[seg * 64 + lane] = x;
-    shB[seg * 64 + lane] = Q;
+    shA[seg * LW + kk] = x;
+    shB[seg * LW + kk] = Q;
     __syncthreads();
     double X = 0.0;
-    for (int g = NW - 1; g > seg; --g) X = shA[g * 64 + lane] + shB[g * 64 + lane] * X;
+    for (int g = G - 1; g > seg; --g) X = shA[g * LW + kk] + shB[g * LW + kk] * X;
     __syncthreads();
     Q = 1.0;
 #pragma unroll
@@ -414,13 +429,12 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
     // z-step R(dt A_z) across lanes (lane = k), column class per line element;
     // neighbours k-1 / k+1 come from lanes -1 / +1 by DPP wave shifts (the
     // window edge lanes keep their own value: they are halo, never output)
-    const TS al[3] = {(TS)a1, (TS)a2, (TS)a3};
     TS ean[KC], ebn[KC];
 #pragma unroll
     for (int q = 0; q < KC; ++q) {
       const int cl = shK[l0 + q];
-      ean[q] = (TS)shZ[(cl * 64 + lane) * 2];
-      ebn[q] = (TS)shZ[(cl * 64 + lane) * 2 + 1];
+      ean[q] = (TS)shZ[(cl * LW + kk) * 2];
+      ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
     }
 #pragma unroll
     for (int r0 = 0; r0 < S; r0 += KC) {
@@ -431,36 +445,40 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line(DevView v, int dir, TS* o
 #pragma unroll
         for (int q = 0; q < KC; ++q) {
           const int cl = shK[l0 + r0 + KC + q];
-          ean[q] = (TS)shZ[(cl * 64 + lane) * 2];
-          ebn[q] = (TS)shZ[(cl * 64 + lane) * 2 + 1];
+          ean[q] = (TS)shZ[(cl * LW + kk) * 2];
+          ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
+        }
+      }
+      // the chunk's KC elements advance through the three stages together
+      // (KC independent chains per stage); the stage loop stays a real loop:
+      // unrolled, the compiler interleaves chunks and the line spills
+      TS u0[KC], w[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { u0[q] = (TS)d[r0 + q]; w[q] = u0[q]; }
+#pragma unroll 1
+      for (int st = 0; st < 3; ++st) {
+        const TS al = st == 0 ? (TS)a1 : (st == 1 ? (TS)a2 : (TS)a3);
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const TS wm = wave_shr1(w[q]);
+          const TS wp = wave_shl1(w[q]);
+          w[q] = u0[q] + al * (eac[q] * (wp - w[q]) - ebc[q] * (w[q] - wm));
         }
       }
 #pragma unroll
-      for (int q = 0; q < KC; ++q) {
-        const TS u0 = (TS)d[r0 + q];
-        TS w = u0;
-        // kept as a real loop: unrolling the stages exposes so much ILP across
-        // elements that the register-resident line spills
-#pragma unroll 1
-        for (int st = 0; st < 3; ++st) {
-          const TS wm = wave_shr1(w);
-          const TS wp = wave_shl1(w);
-          w = u0 + al[st] * (eac[q] * (wp - w) - ebc[q] * (w - wm));
-        }
-        d[r0 + q] = (RT)w;
-      }
+      for (int q = 0; q < KC; ++q) d[r0 + q] = (RT)w[q];
       GW_SCHED_FENCE();
     }
   }
 
   if (MT::expl) {
     // explicit half-step of the line direction: u + c (ea (u+ - u) - eb (u - u-))
-    shA[seg * 64 + lane] = (double)d[0];
-    shB[seg * 64 + lane] = (double)d[S - 1];
+    shA[seg * LW + kk] = (double)d[0];
+    shB[seg * LW + kk] = (double)d[S - 1];
     __syncthreads();
     const TS cc = (TS)c;
-    TS prev = (seg > 0) ? (TS)shB[(seg - 1) * 64 + lane] : (TS)d[0];
-    const TS edge = (l0 + S < N) ? (TS)shA[(seg + 1) * 64 + lane] : (TS)d[S - 1];
+    TS prev = (seg > 0) ? (TS)shB[(seg - 1) * LW + kk] : (TS)d[0];
+    const TS edge = (l0 + S < N) ? (TS)shA[(seg + 1) * LW + kk] : (TS)d[S - 1];
     TS an[KC], bn2[KC];
 #pragma unroll
     for (int q = 0; q < KC; ++q) { an[q] = (TS)ea[l0 + q]; bn2[q] = (TS)eb[l0 + q]; }
@@ -635,7 +653,9 @@ template <class Cfg, class TS, int MODE>
 static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
                         double a3, hipStream_t st) {
   using MT = ModeTraits<MODE>;
-  const int64_t ntiles = MT::halo ? v.htiles_total : (dir == 0 ? v.xtiles_total : v.ytiles_total);
+  constexpr int FAM = Cfg::LW == 32 ? 1 : 0;
+  const int64_t ntiles =
+      MT::halo ? v.htiles_total[FAM] : (dir == 0 ? v.xtiles_total[FAM] : v.ytiles_total[FAM]);
   if (ntiles == 0) return;
   const size_t sh = line_lds_doubles<Cfg, MODE>() * sizeof(double);
   hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
@@ -658,31 +678,53 @@ static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const T
   return hipGetLastError();
 }
 
-// Tile shapes: waves per tile NW, line elements per lane S.
-using Cfg48 = LineCfg<1, 48>;
-using Cfg96 = LineCfg<2, 48>;
-using Cfg192 = LineCfg<4, 48>;
-using Cfg416 = LineCfg<8, 52>;
-using Cfg640 = LineCfg<8, 80>;
+// Tile shapes: waves per tile NW, line elements per lane S, k-window LW.
+// LW = 32 puts two line segments in a wave (one per half-wave): a tile needs
+// half the waves of an LW = 64 tile of the same line, so two tiles share a CU
+// and one computes while the other loads or stores.
+using W64a = LineCfg<1, 48, 64>;
+using W64b = LineCfg<2, 48, 64>;
+using W64c = LineCfg<4, 48, 64>;
+using W64d = LineCfg<8, 52, 64>;
+using W64e = LineCfg<8, 80, 64>;
+using W32a = LineCfg<1, 24, 32>;
+using W32b = LineCfg<1, 48, 32>;
+using W32c = LineCfg<2, 48, 32>;
+using W32d = LineCfg<4, 52, 32>;
+using W32e = LineCfg<8, 40, 32>;
+
+bool line_window_ok(int nmax, int lw) {
+  return nmax >= 1 && ((lw == 64 && nmax <= W64e::NMAX) || (lw == 32 && nmax <= W32e::NMAX));
+}
+
+int line_window_default(int nmax) { return nmax <= W64d::NMAX ? 64 : 32; }
 
 template <class TS>
-static hipError_t line_t(const DevView& v, int mode, int dir, TS* out, const TS* in, double c, double a1,
+static hipError_t line_t(const DevView& v, int mode, int dir, int lw, TS* out, const TS* in, double c, double a1,
                          double a2, double a3, int nmax, hipStream_t st) {
-  if (nmax <= Cfg48::NMAX) return line_cfg<Cfg48>(v, mode, dir, out, in, c, a1, a2, a3, st);
-  if (nmax <= Cfg96::NMAX) return line_cfg<Cfg96>(v, mode, dir, out, in, c, a1, a2, a3, st);
-  if (nmax <= Cfg192::NMAX) return line_cfg<Cfg192>(v, mode, dir, out, in, c, a1, a2, a3, st);
-  if (nmax <= Cfg416::NMAX) return line_cfg<Cfg416>(v, mode, dir, out, in, c, a1, a2, a3, st);
-  if (nmax <= Cfg640::NMAX) return line_cfg<Cfg640>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  if (lw == 64) {
+    if (nmax <= W64a::NMAX) return line_cfg<W64a>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W64b::NMAX) return line_cfg<W64b>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W64c::NMAX) return line_cfg<W64c>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W64d::NMAX) return line_cfg<W64d>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W64e::NMAX) return line_cfg<W64e>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  } else if (lw == 32) {
+    if (nmax <= W32a::NMAX) return line_cfg<W32a>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W32b::NMAX) return line_cfg<W32b>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W32c::NMAX) return line_cfg<W32c>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W32d::NMAX) return line_cfg<W32d>(v, mode, dir, out, in, c, a1, a2, a3, st);
+    if (nmax <= W32e::NMAX) return line_cfg<W32e>(v, mode, dir, out, in, c, a1, a2, a3, st);
+  }
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_line(const DevView& v, int prec, int mode, int dir, void* out, const void* in, double c,
-                       double dt_z, double c3, hipStream_t st) {
+hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, void* out, const void* in,
+                       double c, double dt_z, double c3, hipStream_t st) {
   const int nmax = dir == 0 ? v.nx_max : v.ny_max;
   const double a1 = 2.0 * c3 * dt_z, a2 = 0.5 * dt_z, a3 = dt_z;
   if (prec == 0)
-    return line_t<double>(v, mode, dir, (double*)out, (const double*)in, c, a1, a2, a3, nmax, st);
-  return line_t<float>(v, mode, dir, (float*)out, (const float*)in, c, a1, a2, a3, nmax, st);
+    return line_t<double>(v, mode, dir, lw, (double*)out, (const double*)in, c, a1, a2, a3, nmax, st);
+  return line_t<float>(v, mode, dir, lw, (float*)out, (const float*)in, c, a1, a2, a3, nmax, st);
 }
 
 hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st) {

@@ -37,6 +37,11 @@ typedef enum gw_status {
 
 enum { GW_F64 = 0, GW_F32_STATE = 1 };
 
+/* gw_problem.flags.  The line kernels come in two k-window widths (64 and 32
+ * lanes); by default the first gw_steps call times both for each line
+ * operator of the step and keeps the faster.  These flags force one width. */
+enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2 };
+
 /* Inlet models (src/physics.jl:283-318). */
 enum {
   GW_INLET_CONSTANT = 0,       /* ConstantInlet(T):        T                       */
@@ -90,7 +95,7 @@ typedef struct gw_problem {
   int32_t precision;   /* GW_F64 or GW_F32_STATE */
   void *stream;        /* hipStream_t; NULL: the context creates its own */
   double rock2_c3;     /* cubic coefficient of the minimum-degree ROCK2 polynomial; <= 0: default 1/16 */
-  int32_t flags;       /* reserved, 0 */
+  int32_t flags;       /* GW_FLAG_*, or 0 */
 } gw_problem;
 
 typedef struct gw_ctx gw_ctx;

@@ -19,10 +19,10 @@ def _kw(w):
     return {k: w[k] for k in ("gridx", "gridy", "gridz", "materials", "boreholes", "inlet_model")}
 
 
-def _run(w, nsteps, per_call=None):
+def _run(w, nsteps, per_call=None, line_window=None):
     torch = _torch()
     g = H.gw()
-    cache = g.create_cache(**_kw(w))
+    cache = g.create_cache(line_window=line_window, **_kw(w))
     orc = H.oracle_for(w)
     u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
     u = torch.from_numpy(u0).cuda().reshape(-1).contiguous()
@@ -57,14 +57,18 @@ def _long_line_case(n_target, nbh=2):
                 inlet_model=g.PerWellInlet([14.0, 17.0][:nbh]), dt=120.0, ic=dict(T_surface=2.29, gradient=0.3))
 
 
+LINE_WINDOWS = [64, 32]
+
+
+@pytest.mark.parametrize("line_window", LINE_WINDOWS)
 @pytest.mark.parametrize("n_target", [180, 400, 600])
 @pytest.mark.parametrize("op", ["explicit", "implicit"])
 @pytest.mark.parametrize("direction", [0, 1])
-def test_long_line_operators(n_target, op, direction):
+def test_long_line_operators(n_target, op, direction, line_window):
     torch = _torch()
     g = H.gw()
     w = _long_line_case(n_target)
-    cache = g.create_cache(**_kw(w))
+    cache = g.create_cache(line_window=line_window, **_kw(w))
     orc = H.oracle_for(w)
     u = orc.initial_condition(2.29, 0.3) + np.random.default_rng(7).standard_normal(orc.shape)
     d = torch.from_numpy(u).cuda().reshape(-1).contiguous()
@@ -77,16 +81,17 @@ def test_long_line_operators(n_target, op, direction):
     assert H.rel_err(got, ref) <= 1e-10, (orc.shape, (i, j, k), float(err.max()))
 
 
-@pytest.mark.parametrize("n_target", [40, 180, 400])
+@pytest.mark.parametrize("line_window", LINE_WINDOWS)
+@pytest.mark.parametrize("n_target", [40, 180, 400, 600])
 @pytest.mark.parametrize("mode", [2, 3, 4])
-def test_fused_line_modes(n_target, mode):
+def test_fused_line_modes(n_target, mode, line_window):
     """gw_line_op's fused modes against the oracle's separate operators:
     2 = implicit x then explicit x; 3 = z-step then explicit y;
     4 = implicit y, z-step, explicit y."""
     torch = _torch()
     g = H.gw()
     w = _long_line_case(n_target)
-    cache = g.create_cache(**_kw(w))
+    cache = g.create_cache(line_window=line_window, **_kw(w))
     orc = H.oracle_for(w)
     u = orc.initial_condition(2.29, 0.3) + np.random.default_rng(8).standard_normal(orc.shape)
     src = torch.from_numpy(u).cuda().reshape(-1).contiguous()
@@ -107,22 +112,24 @@ def test_fused_line_modes(n_target, mode):
     assert H.rel_err(got, ref) <= 1e-10, (orc.shape, (i, j, k), float(err.max()))
 
 
+@pytest.mark.parametrize("line_window", LINE_WINDOWS + [None])
 @pytest.mark.parametrize("n_target", [180, 400])
 @pytest.mark.parametrize("nsteps", [1, 2])
-def test_long_line_steps(n_target, nsteps):
+def test_long_line_steps(n_target, nsteps, line_window):
     """One step (z-step + explicit y, x, implicit y) and two steps (adds the
-    fused implicit y + z-step + explicit y)."""
+    fused implicit y + z-step + explicit y); None = autotuned widths."""
     w = _long_line_case(n_target)
-    got, ref, T, T_ref = _run(w, nsteps)
+    got, ref, T, T_ref = _run(w, nsteps, line_window=line_window)
     err = np.abs(got - ref)
     i, j, k = np.unravel_index(np.argmax(err), err.shape)
     assert H.rel_err(got, ref) <= 1e-10, (got.shape, (i, j, k), float(err.max()))
 
 
+@pytest.mark.parametrize("line_window", LINE_WINDOWS)
 @pytest.mark.parametrize("n_target", [90, 180, 400, 600])
-def test_long_lines(n_target):
+def test_long_lines(n_target, line_window):
     w = _long_line_case(n_target)
-    got, ref, T, T_ref = _run(w, 4)
+    got, ref, T, T_ref = _run(w, 4, line_window=line_window)
     assert H.rel_err(got, ref) <= 1e-10, (len(w["gridx"]), len(w["gridy"]))
     assert np.abs(T - T_ref).max() <= 1e-6
 
