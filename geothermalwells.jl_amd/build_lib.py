@@ -1,8 +1,9 @@
 """Builds libgeowells_hip.so for gfx950 in-tree (next to this file).
 
-Two translation units: the set-up / C-ABI host code (compiled without
-floating-point contraction so the material classification and departure
-branches reproduce the reference's FP64 decisions) and the kernels.
+The set-up / C-ABI host code is compiled without floating-point contraction
+(so the material classification and departure branches reproduce the
+reference's FP64 decisions); the kernels are split into the pipe / z-step
+unit and one unit per line-kernel family and precision, compiled in parallel.
 """
 import os
 import subprocess
@@ -23,6 +24,21 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+def _run_all(cmds):
+    """Runs the commands concurrently; raises if any fails."""
+    procs = []
+    for cmd in cmds:
+        print(" ".join(cmd), flush=True)
+        procs.append((cmd, subprocess.Popen(cmd)))
+    failed = [cmd for cmd, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
+
+
+KERNEL_UNITS = ["gw_kernels.hip", "gw_line_w64_f64.hip", "gw_line_w64_f32.hip", "gw_line_w32_f64.hip",
+                "gw_line_w32_f32.hip"]
+
+
 def build(verbose=True):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objdir = os.path.join(HERE, "_build")
@@ -30,11 +46,13 @@ def build(verbose=True):
     common = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
               "-I", CSRC]
     host_obj = os.path.join(objdir, "gw_host.o")
-    kern_obj = os.path.join(objdir, "gw_kernels.o")
-    _run([hipcc, *common, "-ffp-contract=off", "-c", os.path.join(CSRC, "gw_host.cpp"), "-o", host_obj])
-    _run([hipcc, *common, *DEVICE_FLAGS, "-c", os.path.join(CSRC, "gw_kernels.hip"), "-o", kern_obj])
+    kern_objs = [os.path.join(objdir, u.replace(".hip", ".o")) for u in KERNEL_UNITS]
+    cmds = [[hipcc, *common, "-ffp-contract=off", "-c", os.path.join(CSRC, "gw_host.cpp"), "-o", host_obj]]
+    cmds += [[hipcc, *common, *DEVICE_FLAGS, "-c", os.path.join(CSRC, u), "-o", o]
+             for u, o in zip(KERNEL_UNITS, kern_objs)]
+    _run_all(cmds)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, host_obj, kern_obj])
+    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, host_obj, *kern_objs])
     os.replace(tmp, LIB)
     return LIB
 

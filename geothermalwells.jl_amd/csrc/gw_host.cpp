@@ -193,9 +193,11 @@ struct CTab {
   Dep* d_dep = nullptr;
   double* d_means = nullptr;
   RedChunk* d_rchunk = nullptr;
+  RedChunk* d_agroup = nullptr;
   double* d_partial = nullptr;
   int nmeans = 0;
   int nrchunk = 0;
+  int nagroup = 0;
 };
 
 template <class T>
@@ -233,7 +235,8 @@ struct gw_ctx {
   double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
   int nx_max = 0, ny_max = 0, nz_max = 0, countz_max = 0;
   int64_t cells_total = 0, cols_total = 0;
-  int64_t xtiles_total[2] = {}, ytiles_total[2] = {}, htiles_total[2] = {};
+  // tile offsets of the line-length classes, per k-window family
+  int64_t xcls[2][kLineClasses + 1] = {}, ycls[2][kLineClasses + 1] = {}, hcls[2][kLineClasses + 1] = {};
   // k-window width (64 or 32) of each line operator (mode, direction): the
   // length default until the first gw_steps times both widths (autotune), or
   // the width the problem flags force
@@ -266,6 +269,7 @@ struct gw_ctx {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
       (void)hipFree(kv.second.d_rchunk); (void)hipFree(kv.second.d_partial);
+      (void)hipFree(kv.second.d_agroup);
     }
     for (void* p : allocs) (void)hipFree(p);
     if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -276,11 +280,14 @@ struct gw_ctx {
     v.cell_prefix = d_cell_prefix; v.col_prefix = d_col_prefix;
     for (int f = 0; f < 2; ++f) {
       v.xtiles[f] = d_xtiles[f]; v.ytiles[f] = d_ytiles[f]; v.htiles[f] = d_htiles[f];
-      v.xtiles_total[f] = xtiles_total[f]; v.ytiles_total[f] = ytiles_total[f]; v.htiles_total[f] = htiles_total[f];
+      for (int q = 0; q <= kLineClasses; ++q) {
+        v.xcls[f][q] = xcls[f][q]; v.ycls[f][q] = ycls[f][q]; v.hcls[f][q] = hcls[f][q];
+      }
     }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
     if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
-             v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial; }
+             v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial;
+             v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
     v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size();
     v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
     v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
@@ -539,7 +546,7 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   // departure tables
   std::vector<Dep> dep;
   std::vector<BhDev> hb(ctx.nbh_total);
-  std::vector<RedChunk> chunks;
+  std::vector<RedChunk> chunks, groups;
   int gb = 0, mean_off = 0, poff = 0;
   for (size_t si = 0; si < ctx.sc.size(); ++si) {
     const Scen& s = ctx.sc[si];
@@ -627,6 +634,12 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
           poff += B.nlev;
         }
       B.n_red_an = (int)chunks.size() - B.red_an0;
+      // advection groups: kAdvGroup columns of one kind each
+      constexpr int kAdvGroup = 8;
+      for (int c0 = 0; c0 < B.n_in; c0 += kAdvGroup)
+        groups.push_back(RedChunk{gb, 0, B.col_in0 + c0, std::min(kAdvGroup, B.n_in - c0), 0, 0});
+      for (int c0 = 0; c0 < B.n_an; c0 += kAdvGroup)
+        groups.push_back(RedChunk{gb, 1, B.col_an0 + c0, std::min(kAdvGroup, B.n_an - c0), 0, 0});
     }
   }
   try {
@@ -639,6 +652,8 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
     t.nmeans = mean_off;
     t.d_rchunk = dev_upload(chunks);
     t.nrchunk = (int)chunks.size();
+    t.d_agroup = dev_upload(groups);
+    t.nagroup = (int)groups.size();
     std::vector<double> part(std::max(poff, 1), 0.0);
     t.d_partial = dev_upload(part);
   } catch (HipFail& e) {
@@ -836,7 +851,7 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       }
     std::vector<ScenDev> sd;
     std::vector<int64_t> cellp{0}, colp{0};
-    std::vector<LineTile> xt[2], yt[2], ht[2];
+    std::vector<LineTile> xb[2][kLineClasses], yb[2][kLineClasses], hb[2][kLineClasses];
     std::vector<uint16_t> cls;
     std::vector<int32_t> zone, setid;
     std::vector<double> zcoef, inlet_tab;
@@ -855,13 +870,13 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       const int64_t ncell = (int64_t)s.nx * s.ny * s.nz;
       cellp.push_back(cellp.back() + ncell);
       colp.push_back(colp.back() + (int64_t)s.nx * s.ny);
+      const int qx = line_class(s.nx), qy = line_class(s.ny);
+      if (qx < 0 || qy < 0) throw std::runtime_error("grid lines longer than the line kernels support");
       for (int f = 0; f < 2; ++f) {
         const int lw = f == 0 ? 64 : 32;
-        if (line_window_ok(ctx->nx_max, lw)) line_tiles(s, (int)sd.size() - 1, 0, 0, lw, xt[f]);
-        if (line_window_ok(ctx->ny_max, lw)) {
-          line_tiles(s, (int)sd.size() - 1, 1, 0, lw, yt[f]);
-          line_tiles(s, (int)sd.size() - 1, 1, 3, lw, ht[f]);
-        }
+        line_tiles(s, (int)sd.size() - 1, 0, 0, lw, xb[f][qx]);
+        line_tiles(s, (int)sd.size() - 1, 1, 0, lw, yb[f][qy]);
+        line_tiles(s, (int)sd.size() - 1, 1, 3, lw, hb[f][qy]);
       }
       for (size_t b = 0; b < s.bh.size(); ++b)
         inlet_tab.push_back(s.inlet.kind == GW_INLET_PER_WELL ? s.per_well[b] : 0.0);
@@ -869,10 +884,19 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     }
     ctx->cells_total = cellp.back();
     ctx->cols_total = colp.back();
+    // each family's tiles in class order, with the class offsets
+    std::vector<LineTile> xt[2], yt[2], ht[2];
+    auto gather = [](std::vector<LineTile>* b, std::vector<LineTile>& out, int64_t* off) {
+      for (int q = 0; q < kLineClasses; ++q) {
+        off[q] = (int64_t)out.size();
+        out.insert(out.end(), b[q].begin(), b[q].end());
+      }
+      off[kLineClasses] = (int64_t)out.size();
+    };
     for (int f = 0; f < 2; ++f) {
-      ctx->xtiles_total[f] = (int64_t)xt[f].size();
-      ctx->ytiles_total[f] = (int64_t)yt[f].size();
-      ctx->htiles_total[f] = (int64_t)ht[f].size();
+      gather(xb[f], xt[f], ctx->xcls[f]);
+      gather(yb[f], yt[f], ctx->ycls[f]);
+      gather(hb[f], ht[f], ctx->hcls[f]);
     }
     ctx->d_sc = ctx->up(sd);
     ctx->d_cell_prefix = ctx->up(cellp);
@@ -999,11 +1023,9 @@ int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
   if (!rc) rc = hip_status(hipEventCreate(&e1), "event");
   for (int q = 0; q < 4 && !rc; ++q) {
     const int mode = ops[q][0], dir = ops[q][1];
-    const int n = dir == 0 ? ctx.nx_max : ctx.ny_max;
     float best = 0.f;
     int best_lw = ctx.lw_sel[mode][dir];
     for (int lw : {64, 32}) {
-      if (!line_window_ok(n, lw)) continue;
       auto run = [&]() {
         return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream),
                           "line tune");

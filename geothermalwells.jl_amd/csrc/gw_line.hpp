@@ -1,0 +1,471 @@
+// gw_line.hpp -- the line kernel (x/y lines, register-resident, with the
+// fused z-step modes) and its per-family launcher.  Each k-window family and
+// state precision is instantiated in its own translation unit
+// (gw_line_w<LW>_<prec>.hip).
+#pragma once
+
+#include <type_traits>
+
+#include "gw_device.hpp"
+
+namespace gw {
+
+// ---------------------------------------------------------------------------
+// Line kernel.  A tile is a window of LW adjacent k-lines (k = kw .. kw+LW-1)
+// of one x-line (fixed j) or y-line (fixed i); its outputs are the lanes with
+// ko0 <= k < ko1.  The line is cut into G segments of S elements (G*S >= N),
+// each held in registers by LW lanes (one per k): a whole wave when LW = 64,
+// a half-wave when LW = 32 (the wave's two halves hold consecutive segments).
+// Loads and stores of one element index are LW consecutive k values:
+// coalesced.  The Thomas recurrences run segment-locally from zero carry and
+// are then corrected by the carried value times the running product of the
+// multipliers; the carries are a G-step scan in LDS.  The lanes of a window
+// use at most three coefficient sets (one per material zone met in z), staged
+// in LDS and padded with identity rows so the element loops have no branches.
+//
+// Modes (the reference's operators on the same direction, src/solver.jl):
+//   EXPLICIT            u <- (I + cA) u                               (:34-58)
+//   IMPLICIT            u <- (I - cA)^-1 u                            (:111-232)
+//   IMPLICIT_EXPLICIT   u <- (I + cA)(I - cA)^-1 u   x-lines, :450 + :455
+//   ZY                  u <- (I + cA_y) R(dt A_z) u   next step's z-step and
+//                       explicit y (:23-31 under ROCK2, then :444)
+//   YZY                 u <- (I + cA_y) R(dt A_z) (I - cA_y)^-1 u      (:461, then
+//                       the next step's z-step and explicit y)
+// ZY / YZY run on y-lines with a 3-lane halo in k: the z-stencil is applied
+// three times across lanes (wave-wide DPP shifts), so only lanes 3..LW-4 of a
+// window are outputs; the halo lanes re-solve their own lines.  A shift that
+// crosses from one half-wave into the other (LW = 32) reaches a halo lane
+// only, and in three stages its value travels at most three lanes: the
+// outputs never see it.
+// ---------------------------------------------------------------------------
+template <int NW_, int S_, int LW_>
+struct LineCfg {
+  static constexpr int NW = NW_, S = S_;
+  static constexpr int LW = LW_;             // k-window width: lanes per line element
+  static constexpr int H = 64 / LW_;         // line segments per wave (one per half-wave when LW = 32)
+  static constexpr int G = NW_ * H;          // line segments per tile
+  static constexpr int NMAX = G * S_;        // longest line
+  static constexpr int THREADS = NW_ * 64;
+  static_assert(LW_ == 64 || LW_ == 32, "k-window of a full or half wave");
+};
+
+constexpr int kSlots = 3;      // coefficient sets per window
+constexpr int kZcMax = 16;     // column classes staged for the fused z-step
+
+template <int MODE> struct ModeTraits {
+  static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY;
+  static constexpr bool impl = MODE == LINE_IMPLICIT || MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_YZY;
+  static constexpr bool expl = MODE != LINE_IMPLICIT;
+  // coefficient arrays staged per set slot: bd, fw, bu (implicit) then ea, eb
+  // (explicit), each NMAX long
+  static constexpr int NA = (impl ? 3 : 0) + (expl ? 2 : 0);
+  static constexpr int A_BD = 0, A_FW = 1, A_BU = 2, A_EA = impl ? 3 : 0, A_EB = A_EA + 1;
+};
+
+template <class Cfg, int MODE>
+constexpr size_t line_lds_doubles() {
+  using MT = ModeTraits<MODE>;
+  return 2 * Cfg::G * Cfg::LW + kSlots * MT::NA * Cfg::NMAX +
+         (MT::halo ? kZcMax * Cfg::LW * 2 + (Cfg::NMAX + 1) / 2 : 0);
+}
+
+GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous range of tiles so neighbouring k-windows of a line share an L2.
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = b % 8, i = b / 8;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+template <class Cfg, class TS, int MODE>
+__global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS* out, const TS* in,
+                                                          double c, double a1, double a2, double a3,
+                                                          int64_t tile0, int64_t ntiles) {
+  using MT = ModeTraits<MODE>;
+  // The line is held in registers in the storage type; every recurrence runs
+  // in FP64 and rounds to the storage type only when it writes an element.
+  using RT = TS;
+  constexpr int S = Cfg::S, NM = Cfg::NMAX, LW = Cfg::LW, H = Cfg::H, G = Cfg::G;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW] carries / edges
+  double* shB = shA + G * LW;                          // [G][LW]
+  double* shC = shB + G * LW;                          // [kSlots][MT::NA][NM] coefficient arrays
+  double* shZ = shC + kSlots * MT::NA * NM;            // [kZcMax][LW][2]: z coefficients
+  int* shK = reinterpret_cast<int*>(shZ + kZcMax * LW * 2);  // [NM] column class per line element
+
+  constexpr int FAM = LW == 32 ? 1 : 0;
+  const LineTile* tiles = MT::halo ? v.htiles[FAM] : (dir == 0 ? v.xtiles[FAM] : v.ytiles[FAM]);
+  const LineTile T = tiles[tile0 + xcd_remap(blockIdx.x, ntiles)];
+  const ScenDev S_ = v.sc[T.scen];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int kk = lane % LW;    // k offset in the window
+  const int hh = lane / LW;    // half-wave (0 when LW = 64)
+  const int seg = wv * H + hh;
+  const int N = dir == 0 ? S_.nx : S_.ny;
+  const int nz = S_.nz;
+  const int k = T.kw + kk;
+  const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
+  const int kmin = T.kw < 0 ? 0 : T.kw;
+  // lane byte offset from the wave's first segment: the k offset plus, for
+  // the upper half-wave, S line elements (below 2^32: nx, ny <= 640,
+  // nz <= 4096, S <= 52)
+  const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(TS) +
+                      (unsigned)(hh * S) * (unsigned)(dir == 0 ? nz * S_.ny : nz) * (unsigned)sizeof(TS);
+  const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
+  const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
+
+  // coefficient sets of the window's lanes, identity rows beyond N
+  const int32_t* sids = v.setid + (dir == 0 ? S_.o_xset : S_.o_yset) + (int64_t)T.line * S_.nzone;
+  const int sid = sids[v.zone[S_.o_zone + kc]];
+  const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
+  for (int q = 0; q < T.nset; ++q) {
+    const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : T.set[2]);
+    const double* g = v.coef + v.set_off[sq];
+    double* dst = shC + q * MT::NA * NM;
+    for (int l = tid; l < NM; l += Cfg::THREADS) {
+      const bool ok = l < N;
+      if (MT::impl) {
+        dst[MT::A_BD * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
+        dst[MT::A_FW * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
+        dst[MT::A_BU * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
+      }
+      if (MT::expl) {
+        dst[MT::A_EA * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
+        dst[MT::A_EB * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
+      }
+    }
+  }
+  if (MT::halo) {
+    const double* zc = v.zcoef + 2 * S_.o_zcoef;
+    const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
+    for (int q = wv; q < ncls; q += Cfg::NW) {
+      shZ[(q * LW + kk) * 2 + hh] = zc[2 * ((int64_t)q * nz + kc) + hh];
+      if (H == 1) shZ[(q * LW + kk) * 2 + 1] = zc[2 * ((int64_t)q * nz + kc) + 1];
+    }
+    const uint16_t* clsl = v.cls + S_.o_cls + (int64_t)S_.ny * T.line;
+    for (int l = tid; l < NM; l += Cfg::THREADS) shK[l] = clsl[l < N ? l : N - 1];
+  }
+
+  RT d[S];
+  const int l0 = seg * S;
+  const int64_t ustride = uniform64(stride);
+  const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
+  const TS* pin = in + useg;
+  if (l0 + S <= N) {
+#pragma unroll
+    for (int r = 0; r < S; ++r) d[r] = (RT)*lane_ptr(pin + (int64_t)r * ustride, kb);
+  } else {
+#pragma unroll
+    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? (RT)*lane_ptr(pin + (int64_t)r * ustride, kb) : (RT)0;
+  }
+  __syncthreads();
+
+  const double* cs = shC + slot * MT::NA * NM;
+  const double* bd = cs + MT::A_BD * NM;
+  const double* fw = cs + MT::A_FW * NM;
+  const double* bu = cs + MT::A_BU * NM;
+  const double* ea = cs + MT::A_EA * NM;
+  const double* eb = cs + MT::A_EB * NM;
+
+  // Every element loop below runs in chunks of KC elements: the LDS operands
+  // of the next chunk are fetched while the current one computes.
+  constexpr int KC = 4;
+  static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
+  if (MT::impl) {
+    // forward: y_l = b_l - fw_l y_{l-1}
+    auto wmul = [&](int l) { return fw[l]; };
+    double y = 0.0, P = 1.0, wn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + q);
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + r0 + KC + q);
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        y = (double)d[r0 + q] - wc[q] * y;
+        P = -wc[q] * P;
+        d[r0 + q] = y;
+      }
+    }
+    shA[seg * LW + kk] = y;
+    shB[seg * LW + kk] = P;
+    __syncthreads();
+    double C = 0.0;
+    for (int g = 0; g < seg; ++g) C = shA[g * LW + kk] + shB[g * LW + kk] * C;
+    __syncthreads();
+    P = 1.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + q);
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = wmul(l0 + r0 + KC + q);
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        P = -wc[q] * P;
+        d[r0 + q] = (double)d[r0 + q] + P * C;
+      }
+    }
+    // backward: x_l = bd_l y_l - bu_l x_{l+1}
+    double x = 0.0, Q = 1.0, bn[KC], un[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int l = l0 + S - 1 - q;
+      bn[q] = bd[l];
+      un[q] = bu[l];
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double bc[KC], uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { bc[q] = bn[q]; uc[q] = un[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const int l = l0 + S - 1 - (r0 + KC + q);
+          bn[q] = bd[l];
+          un[q] = bu[l];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        x = bc[q] * (double)d[r] - uc[q] * x;
+        Q = -uc[q] * Q;
+        d[r] = x;
+      }
+    }
+    shA[seg * LW + kk] = x;
+    shB[seg * LW + kk] = Q;
+    __syncthreads();
+    double X = 0.0;
+    for (int g = G - 1; g > seg; --g) X = shA[g * LW + kk] + shB[g * LW + kk] * X;
+    __syncthreads();
+    Q = 1.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) uc[q] = un[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - (r0 + KC + q)];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        Q = -uc[q] * Q;
+        d[r] = (double)d[r] + Q * X;
+      }
+    }
+  }
+
+  if (MT::halo) {
+    // z-step R(dt A_z) across lanes (lane = k), column class per line element;
+    // neighbours k-1 / k+1 come from lanes -1 / +1 by DPP wave shifts (the
+    // window edge lanes keep their own value: they are halo, never output)
+    TS ean[KC], ebn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const int cl = shK[l0 + q];
+      ean[q] = (TS)shZ[(cl * LW + kk) * 2];
+      ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      TS eac[KC], ebc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { eac[q] = ean[q]; ebc[q] = ebn[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const int cl = shK[l0 + r0 + KC + q];
+          ean[q] = (TS)shZ[(cl * LW + kk) * 2];
+          ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
+        }
+      }
+      // the chunk's KC elements advance through the three stages together
+      // (KC independent chains per stage); the stage loop stays a real loop:
+      // unrolled, the compiler interleaves chunks and the line spills
+      TS u0[KC], w[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { u0[q] = (TS)d[r0 + q]; w[q] = u0[q]; }
+#pragma unroll 1
+      for (int st = 0; st < 3; ++st) {
+        const TS al = st == 0 ? (TS)a1 : (st == 1 ? (TS)a2 : (TS)a3);
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          const TS wm = wave_shr1(w[q]);
+          const TS wp = wave_shl1(w[q]);
+          w[q] = u0[q] + al * (eac[q] * (wp - w[q]) - ebc[q] * (w[q] - wm));
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) d[r0 + q] = (RT)w[q];
+    }
+  }
+
+  if (MT::expl) {
+    // explicit half-step of the line direction: u + c (ea (u+ - u) - eb (u - u-))
+    shA[seg * LW + kk] = (double)d[0];
+    shB[seg * LW + kk] = (double)d[S - 1];
+    __syncthreads();
+    const TS cc = (TS)c;
+    TS prev = (seg > 0) ? (TS)shB[(seg - 1) * LW + kk] : (TS)d[0];
+    const TS edge = (l0 + S < N) ? (TS)shA[(seg + 1) * LW + kk] : (TS)d[S - 1];
+    TS an[KC], bn2[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) { an[q] = (TS)ea[l0 + q]; bn2[q] = (TS)eb[l0 + q]; }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      TS ac[KC], bc2[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { ac[q] = an[q]; bc2[q] = bn2[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) { an[q] = (TS)ea[l0 + r0 + KC + q]; bn2[q] = (TS)eb[l0 + r0 + KC + q]; }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = r0 + q;
+        const TS x = (TS)d[r];
+        const TS xp = (r < S - 1) ? (TS)d[r + 1] : edge;
+        const TS o = x + cc * (ac[q] * (xp - x) - bc2[q] * (x - prev));
+        prev = x;
+        d[r] = (RT)o;
+      }
+    }
+  }
+
+  if (k >= T.ko0 && k < T.ko1) {
+    TS* pout = out + useg;
+    if (l0 + S <= N) {
+#pragma unroll
+      for (int r = 0; r < S; ++r) *lane_ptr(pout + (int64_t)r * ustride, kb) = (TS)d[r];
+    } else {
+#pragma unroll
+      for (int r = 0; r < S; ++r)
+        if (l0 + r < N) *lane_ptr(pout + (int64_t)r * ustride, kb) = (TS)d[r];
+    }
+  }
+}
+
+
+template <class Cfg, class TS, int MODE>
+static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
+                        double a3, int64_t tile0, int64_t ntiles, hipStream_t st) {
+  const size_t sh = line_lds_doubles<Cfg, MODE>() * sizeof(double);
+  hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
+                     c, a1, a2, a3, tile0, ntiles);
+}
+
+template <class Cfg, class TS>
+static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const TS* in, double c, double a1,
+                           double a2, double a3, int64_t tile0, int64_t ntiles, hipStream_t st) {
+  switch (mode) {
+    case LINE_EXPLICIT: line_launch<Cfg, TS, LINE_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
+    case LINE_IMPLICIT: line_launch<Cfg, TS, LINE_IMPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
+    case LINE_IMPLICIT_EXPLICIT:
+      line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st);
+      break;
+    case LINE_ZY: line_launch<Cfg, TS, LINE_ZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
+    case LINE_YZY: line_launch<Cfg, TS, LINE_YZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Tile shapes: waves per tile NW, line elements per lane S, k-window LW.
+// LW = 32 puts two line segments in a wave (one per half-wave): a tile needs
+// half the waves of an LW = 64 tile of the same line, so two tiles share a CU
+// and one computes while the other loads or stores.
+
+// Tile shapes of one k-window family, one per line-length class
+// (kLineClassMax): waves per tile NW, line elements per lane S.  LW = 32 puts
+// two line segments in a wave (one per half-wave), so a tile needs half the
+// waves of an LW = 64 tile of the same line and two tiles share a CU: one
+// computes while the other loads or stores.
+template <int LW> struct LineFamily;
+template <> struct LineFamily<64> {
+  using C0 = LineCfg<1, 48, 64>;
+  using C1 = LineCfg<2, 32, 64>;
+  using C2 = LineCfg<2, 48, 64>;
+  using C3 = LineCfg<4, 32, 64>;
+  using C4 = LineCfg<4, 40, 64>;
+  using C5 = LineCfg<4, 48, 64>;
+  using C6 = LineCfg<8, 32, 64>;
+  using C7 = LineCfg<8, 40, 64>;
+  using C8 = LineCfg<8, 52, 64>;
+  using C9 = LineCfg<8, 80, 64>;
+};
+template <> struct LineFamily<32> {
+  using C0 = LineCfg<1, 24, 32>;
+  using C1 = LineCfg<1, 32, 32>;
+  using C2 = LineCfg<1, 48, 32>;
+  using C3 = LineCfg<2, 32, 32>;
+  using C4 = LineCfg<2, 40, 32>;
+  using C5 = LineCfg<2, 48, 32>;
+  using C6 = LineCfg<4, 32, 32>;
+  using C7 = LineCfg<4, 40, 32>;
+  using C8 = LineCfg<4, 52, 32>;
+  using C9 = LineCfg<8, 40, 32>;
+};
+
+template <int LW>
+constexpr bool family_matches_classes() {
+  using F = LineFamily<LW>;
+  return F::C0::NMAX == kLineClassMax[0] && F::C1::NMAX == kLineClassMax[1] && F::C2::NMAX == kLineClassMax[2] &&
+         F::C3::NMAX == kLineClassMax[3] && F::C4::NMAX == kLineClassMax[4] && F::C5::NMAX == kLineClassMax[5] &&
+         F::C6::NMAX == kLineClassMax[6] && F::C7::NMAX == kLineClassMax[7] && F::C8::NMAX == kLineClassMax[8] &&
+         F::C9::NMAX == kLineClassMax[9];
+}
+static_assert(kLineClasses == 10, "one tile shape per line-length class");
+static_assert(family_matches_classes<64>() && family_matches_classes<32>(), "tile shapes follow the classes");
+
+// One launch per non-empty line-length class, each with its own tile shape.
+template <int LW, class TS>
+hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const void* inp, double c, double a1,
+                       double a2, double a3, hipStream_t st) {
+  using F = LineFamily<LW>;
+  TS* out = static_cast<TS*>(outp);
+  const TS* in = static_cast<const TS*>(inp);
+  const bool halo = mode == LINE_ZY || mode == LINE_YZY;
+  const int f = LW == 32 ? 1 : 0;
+  const int64_t* cls = halo ? v.hcls[f] : (dir == 0 ? v.xcls[f] : v.ycls[f]);
+  for (int q = 0; q < kLineClasses; ++q) {
+    const int64_t t0 = cls[q], n = cls[q + 1] - cls[q];
+    if (n == 0) continue;
+    hipError_t e;
+    switch (q) {
+      case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace gw

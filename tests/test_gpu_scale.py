@@ -151,3 +151,30 @@ def test_array_3x3_prefix():
     got, ref, T, T_ref = _run(w, 2)
     assert H.rel_err(got, ref) <= 1e-10
     assert np.abs(T - T_ref).max() <= 1e-6
+
+
+@pytest.mark.parametrize("line_window", LINE_WINDOWS + [None])
+def test_batched_line_classes(line_window):
+    """One batch with a scenario in every line-length class (each class
+    launches its own tile shape), two steps, against the oracle per scenario."""
+    torch = _torch()
+    g = H.gw()
+    ws = [_long_line_case(n, nbh=1) for n in (40, 60, 90, 120, 150, 180, 240, 300, 400, 600)]
+    classes = {next(q for q, m in enumerate((48, 64, 96, 128, 160, 192, 256, 320, 416, 640)) if len(w["gridx"]) <= m)
+               for w in ws}
+    assert classes == set(range(10)), classes
+    cache = g.create_batch_cache([_kw(w) for w in ws], line_window=line_window)
+    u = cache.new_state()
+    refs = []
+    for s, w in enumerate(ws):
+        orc = H.oracle_for(w)
+        u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+        cache.scenario_view(u, s).copy_(torch.from_numpy(u0))
+        refs.append((orc, u0))
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 120.0, 2))
+    T = cache.T_outlet
+    for s, (orc, u0) in enumerate(refs):
+        ref, hist, _ = orc.run(u0, 0.0, 120.0, 2, cache.rock2_c3)
+        got = cache.scenario_view(u, s).cpu().numpy()
+        assert H.rel_err(got, ref) <= 1e-10, (s, got.shape)
+        assert abs(T[s] - hist[-1][0]) <= 1e-6, s
