@@ -129,6 +129,37 @@ def bench_batched(gw, steps, warmup, err_steps):
     return res
 
 
+def bench_large(gw, steps, warmup, err_steps):
+    """The 5x5 array (601 x 601 x 2002 cells, dt 600 s): FP32 state (FP64
+    Thomas elimination and outlet reductions) against FP64, cell-steps/s of
+    each and the FP32 outlet error after err_steps steps from the same
+    initial field."""
+    import torch
+    from geothermalwells_jl_amd import workloads as W
+    w = W.c4()
+    cells = W.cells(w)
+    dt = w["dt"]
+    res = {"workload": "5x5 borehole array, 601x601x2002 cells, dt 600 s, per-well inlets", "cells": cells,
+           "steps": steps, "unit": "cell-steps/s"}
+    outs = {}
+    n_err = max(err_steps, warmup + steps)
+    for prec, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        cache = gw.create_cache(precision=prec, **W.scenario_kwargs(w))
+        u = _init_state(gw, cache, [w], dtype)
+        t = _timed_steps(gw, cache, u, dt, warmup, steps)
+        res[prec] = cells * steps / t
+        rest = n_err - warmup - steps
+        if rest > 0:
+            gw._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), (warmup + steps) * dt, dt, rest))
+        outs[prec] = cache.T_outlet
+        del cache, u
+        torch.cuda.empty_cache()
+    res["speedup_f32_vs_f64"] = res["f32"] / res["f64"]
+    res["outlet_err_after_steps"] = n_err
+    res["outlet_max_abs_err_fp32_vs_fp64_K"] = float(np.abs(outs["f32"] - outs["f64"]).max())
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +173,10 @@ def main():
                     help="timed steps of the 64-scenario batched comparison (0: skip)")
     ap.add_argument("--batched-err-steps", type=int, default=720,
                     help="steps after which the FP32 outlet error of the batched comparison is taken")
+    ap.add_argument("--large-steps", type=int, default=20,
+                    help="timed steps of the 601x601x2002 FP32-vs-FP64 comparison (0: skip)")
+    ap.add_argument("--large-err-steps", type=int, default=144,
+                    help="steps after which the FP32 outlet error of that comparison is taken")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -236,6 +271,11 @@ def main():
         del cache, u
         torch.cuda.empty_cache()
         out["batched"] = bench_batched(gw, args.batched_steps, 3, args.batched_err_steps)
+    if args.large_steps > 0:
+        if "cache" in locals():
+            del cache, u
+        torch.cuda.empty_cache()
+        out["large_fp32"] = bench_large(gw, args.large_steps, 2, args.large_err_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads, args.cpu_steps)
     print(json.dumps(out), flush=True)
