@@ -47,6 +47,28 @@ def pipe_bytes(cache, w, s):
     return 2 * ((n_in + n_an) * countz * 2 * s + n_in * n_out * s + n_an * countz * s)
 
 
+PROFILE = os.path.join(ROOT, "profiles", "c1_3x3_fp64_step_kernels.md")
+
+
+def profile_traffic(mode):
+    """HBM bytes per launch of the steady-state line kernel of `mode` (the row
+    with the most calls) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload, with its kernel name and trace duration."""
+    if not os.path.exists(PROFILE):
+        return None
+    best = None
+    for line in open(PROFILE):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) < 9 or not cells[1].startswith("`void k_line<") or not cells[1].endswith(f"double, {mode}>`"):
+            continue
+        calls, avg_us, rd, wr = int(cells[2]), float(cells[3]), float(cells[5]), float(cells[6])
+        if best is None or calls > best[1]:
+            best = (cells[1].strip("`"), calls, avg_us, (rd + wr) * 2 ** 20)
+    if best is None:
+        return None
+    return {"kernel": best[0], "trace_avg_us": best[2], "bytes_per_launch": best[3]}
+
+
 def cpu_baseline(w, threads, steps):
     """The oracle (a restatement of the reference CPU path that keeps its
     per-cell structure) on the host cores, on a bounded sample."""
@@ -236,6 +258,7 @@ def main():
     dom = max(field_ops, key=lambda k: per_op[k])
     dom_bytes = cells * 2 * s  # one read + one write of the state per launch
     achieved = dom_bytes / (per_op[dom] * 1e-3)
+    traffic = profile_traffic({"implicit_explicit_x": 2, "yzy": 4}[dom])
     B_cs = 10 * s
     Bp = pipe_bytes(cache, w, s)
     steps_per_s = args.steps / wall
@@ -258,7 +281,10 @@ def main():
         "config": {"workload": "3x3 borehole array, 401x401x1002 cells, FP64, dt 300 s, per-well inlets",
                    "cells": cells, "boreholes": len(w["boreholes"]), "dt_s": dt, "graph_per_step": True},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_source": (f"{os.path.relpath(PROFILE, ROOT)}: {traffic['kernel']} "
+                                        f"(trace avg {traffic['trace_avg_us']:.1f} us)") if traffic else None,
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": per_op[dom]},
         "step_roofline": {"B_cs_5sweep": B_cs, "pipe_bytes_per_step": Bp,
                           "frac_5sweep": step_bw / HBM_PEAK, "frac_2sweep_as_run": fused_bw / HBM_PEAK},

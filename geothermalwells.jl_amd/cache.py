@@ -274,6 +274,7 @@ def save_and_print_callback(saveat, print_every_n=1000, write_to_jld=False, data
             print(f"Step {n}, t = {t} s, or {round(t / 3600, 2)} hours", flush=True)
 
     print_cb = DiscreteCallback(lambda u, t, integ: integ.stats.naccept % print_every_n == 0, print_affect)
+    print_cb.every_n = print_every_n  # lets the fused driver find the next printing step
     saved_values = SavedValues(Float_used_to_save)
     counter = [0]
 
@@ -300,4 +301,5 @@ def get_simulation_callback(*, saveat, print_every_n=1000, write_to_jld=False, d
                                                               write_to_jld=write_to_jld,
                                                               data_folder_dir=data_folder_dir)
     adi = DiscreteCallback(lambda u, t, integ: True, ADI_and_ADV_callback_)
+    adi.every_step_adi = True  # the fused driver runs it inside gw_steps
     return CallbackSet(adi, print_cb, save_cb), saved_values

@@ -118,9 +118,38 @@ def _rock2_step(integ):
     integ.stats.nf += 3
 
 
+def _fused_driver_ok(alg, cbs):
+    """The whole-step fast path applies when the callbacks are the ones
+    get_simulation_callback builds: the every-step ADI/advection callback,
+    the printing callback and the saving callback."""
+    if not alg.fused:
+        return False
+    n_adi = sum(1 for cb in cbs if getattr(cb, "every_step_adi", False))
+    known = all(getattr(cb, "every_step_adi", False) or hasattr(cb, "every_n") or isinstance(cb, SavingCallback)
+                for cb in cbs)
+    return known and n_adi == 1
+
+
+def _next_event(cbs, n, nsteps, t0, dt):
+    """First step index > n at which a printing or saving callback acts."""
+    nxt = nsteps
+    for cb in cbs:
+        if hasattr(cb, "every_n"):
+            nxt = min(nxt, (n // cb.every_n + 1) * cb.every_n)
+        elif isinstance(cb, SavingCallback):
+            for s_t in cb.saveat:
+                k = int(round((s_t - t0) / dt))
+                if n < k < nxt and abs(t0 + k * dt - s_t) <= 1e-9 * max(1.0, abs(s_t)):
+                    nxt = k
+    return nxt
+
+
 def solve(prob, alg, *, callback=None, dt, adaptive=False, save_everystep=False, maxiters=int(1e10), **_):
     """Fixed-step time loop: ROCK2 z-step, then every callback in order
-    (DiscreteCallback affects, SavingCallback at step-aligned save times)."""
+    (DiscreteCallback affects, SavingCallback at step-aligned save times).
+    With ROCK2(fused=True) and the callbacks of get_simulation_callback, the
+    steps between printing/saving events run as one gw_steps call (the fused
+    step pipeline, one captured graph per step)."""
     if adaptive:
         raise NotImplementedError("only adaptive=false is supported (as in the reference examples)")
     integ = Integrator(prob, alg, float(dt))
@@ -129,8 +158,24 @@ def solve(prob, alg, *, callback=None, dt, adaptive=False, save_everystep=False,
         if isinstance(cb, SavingCallback):
             cb.maybe_save(integ)
     t0, t1 = prob.tspan
-    nsteps = int(round((t1 - t0) / dt))
-    for n in range(min(nsteps, maxiters)):
+    nsteps = min(int(round((t1 - t0) / dt)), maxiters)
+    if _fused_driver_ok(alg, cbs):
+        p = integ.p
+        n = 0
+        while n < nsteps:
+            nxt = _next_event(cbs, n, nsteps, t0, dt)
+            _abi.check(p._lib.gw_steps(p.ctx, _ptr(p, integ.u), float(t0 + n * dt), float(dt), nxt - n))
+            n = nxt
+            integ.t = t0 + n * dt
+            integ.tprev = integ.t - dt
+            integ.stats.naccept = n
+            for cb in cbs:
+                if isinstance(cb, SavingCallback):
+                    cb.maybe_save(integ)
+                elif not getattr(cb, "every_step_adi", False) and cb.condition(integ.u, integ.t, integ):
+                    cb.affect(integ)
+        return integ
+    for n in range(nsteps):
         integ.tprev = integ.t
         _rock2_step(integ)
         integ.t = t0 + (n + 1) * dt

@@ -119,14 +119,20 @@ int gw_adi_adv(gw_ctx *ctx, void *phi, void *scratch, double t, double dt);
 /* The ROCK2 z-step of one step at its minimum degree, fused:
  * phi = R(dt*A_z) phi with R(z) = 1 + z + z^2/2 + c3 z^3 (the polynomial the
  * examples' ROCK2(eigen_est=eigen_estimator) applies, src/solver.jl:23-31,
- * 495).  GW_EINVAL when dt*eigen_estimate leaves the minimum degree. */
+ * 495).  GW_EINVAL when dt times the Gershgorin bound of A_z leaves the
+ * polynomial's real stability interval. */
 int gw_zstep(gw_ctx *ctx, void *phi, double dt);
 
 /* One full step from t to t+dt: gw_zstep then gw_adi_adv(t+dt). */
 int gw_step(gw_ctx *ctx, void *phi, double t, double dt);
 
-/* nsteps full steps from t; replays one captured hipGraph per step when the
- * inlet models allow it (every kind but GW_INLET_HOST). */
+/* nsteps full steps from t as the fused step pipeline (two full-field sweeps
+ * per step: implicit+explicit x, and implicit y fused with the next step's
+ * z-step and explicit y), with the arithmetic of gw_zstep + gw_adi_adv.
+ * Replays one captured hipGraph per step when the inlet models allow it (every
+ * kind but GW_INLET_HOST).  The first call times the line kernels' two
+ * k-window widths (unless gw_problem.flags forces one); phi is only read by
+ * that.  The result is in phi. */
 int gw_steps(gw_ctx *ctx, void *phi, double t, double dt, int32_t nsteps);
 
 /* cache.T_outlet after the last advection sub-step (src/solver.jl:253-257):

@@ -165,6 +165,31 @@ def test_rock2_driver_matches_fused():
     torch.cuda.synchronize()
 
 
+def test_fused_driver_matches_stage_driver():
+    """solve(ROCK2(fused=True)) with the simulation callbacks runs gw_steps
+    between save points; the saved snapshots and the outlet equal the
+    reference call pattern's (ROCK2 stages + callback every step)."""
+    g = H.gw()
+    w = H.small_case(nbh=2, inlet="hx")
+    cache, orc, u0, dt = _setup(w)
+    out = {}
+    for fused in (False, True):
+        cb, saved = g.get_simulation_callback(saveat=[0.0, 120.0, 300.0, 360.0], print_every_n=4)
+        u = _dev(u0, dt)
+        prob = g.ODEProblem(g.rhs_diffusion_z_, u, (0.0, 360.0), cache)
+        integ = g.solve(prob, g.ROCK2(max_stages=100, eigen_est=g.eigen_estimator, fused=fused), callback=cb,
+                        dt=60.0, adaptive=False)
+        out[fused] = (list(saved.t), [np.asarray(x, dtype=np.float64) for x in saved.saveval],
+                      cache.T_outlet.copy(), integ.t, integ.stats.naccept)
+    (ta, va, Ta, tea, na), (tb, vb, Tb, teb, nb) = out[False], out[True]
+    assert ta == tb == [0.0, 120.0, 300.0, 360.0] and tea == teb == 360.0 and na == nb == 6
+    for a, b in zip(va, vb):
+        assert H.rel_err(b, a) <= 1e-6  # snapshots are stored in Float32
+    assert np.abs(Ta - Tb).max() <= OUTLET_TOL
+    ref, hist, _ = orc.run(u0, 0.0, 60.0, 6, cache.rock2_c3)
+    assert np.abs(Tb - hist[-1]).max() <= OUTLET_TOL
+
+
 def test_golden_example_gpu():
     """examples/single_borehole_basic.jl end to end on the GPU: the reference's
     golden outlet (test/test_integration.jl:17) and its invariants."""
