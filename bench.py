@@ -182,6 +182,90 @@ def bench_large(gw, steps, warmup, err_steps):
     return res
 
 
+def _objective(gw, cache, u0, dt, n):
+    """J = mean over steps of the mean outlet, by the forward steps alone (one
+    gw_steps call and one outlet read per step)."""
+    import torch
+    u = u0.clone()
+    torch.cuda.synchronize()
+    ptr = cache.check_state(u)
+    acc = 0.0
+    for s in range(n):
+        gw._abi.check(cache._lib.gw_steps(cache.ctx, ptr, s * dt, dt, 1))
+        acc += float(cache.T_outlet.mean())
+    return acc / n
+
+
+def bench_adjoint(gw, steps, seg, n_fd):
+    """The layered single borehole (121 x 121 x 602 cells, 20 rock layers,
+    dt 300 s): gradient of the mean outlet over `steps` steps with respect to
+    the 20 layer conductivities and the inlet temperature, checkpointed every
+    `seg` steps; its cost against the forward steps alone, and central
+    differences (h = 1e-6 relative) of the inlet and the n_fd layers of
+    largest |dJ/dk|."""
+    import torch
+    from geothermalwells_jl_amd import workloads as W
+    w = W.c3()
+    cells = W.cells(w)
+    dt = w["dt"]
+    kw = W.scenario_kwargs(w)
+    cache = gw.create_cache(precision="f64", **kw)
+    u0 = _init_state(gw, cache, [w], torch.float64)
+    lib = cache._lib
+    u = u0.clone()
+    torch.cuda.synchronize()
+    gw._abi.check(lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, dt, 8))  # tuning, graphs
+    cache.sync()
+    u.copy_(u0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    gw._abi.check(lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, dt, steps))
+    cache.sync()
+    t_fwd = time.perf_counter() - t
+    gw.outlet_gradient(cache, u0, 0.0, dt, 2, checkpoint_every=seg)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    J, dk, dT = gw.outlet_gradient(cache, u0, 0.0, dt, steps, checkpoint_every=seg)
+    t_grad = time.perf_counter() - t
+    res = {"workload": "single borehole, 20 rock layers, 121x121x602 cells, FP64, dt 300 s, constant inlet",
+           "cells": cells, "steps": steps, "checkpoint_every": seg, "J_K": J,
+           "forward_ms_per_step": 1e3 * t_fwd / steps, "gradient_ms_per_step": 1e3 * t_grad / steps,
+           "gradient_cost_in_forward_steps": t_grad / t_fwd,
+           "dJ_dk": [float(x) for x in dk], "dJ_dTin": float(dT.sum())}
+    del cache, u
+    mats = kw["materials"]
+    fd = []
+    h = 1e-6
+
+    def J_of(materials=mats, T_in=None):
+        k2 = dict(kw, materials=materials)
+        if T_in is not None:
+            k2["inlet_model"] = gw.ConstantInlet(T_in)
+        c = gw.create_cache(precision="f64", **k2)
+        return _objective(gw, c, u0, dt, steps)
+
+    def layered(q, f):
+        kl = list(mats.k_rock_layers)
+        kl[q] *= f
+        return gw.StratifiedMaterialProperties(kl, mats.rho_c_rock_layers, mats.layer_depths, mats.k_water,
+                                               mats.rho_c_water, mats.k_steel, mats.rho_c_steel,
+                                               mats.k_insulating, mats.rho_c_insulating, mats.k_backfill,
+                                               mats.rho_c_backfill)
+
+    T0 = kw["inlet_model"].T_const
+    d = (J_of(T_in=T0 * (1 + h)) - J_of(T_in=T0 * (1 - h))) / (2 * h * T0)
+    fd.append({"param": "T_in", "adjoint": float(dT.sum()), "central_difference": d})
+    for q in np.argsort(-np.abs(dk))[:n_fd]:
+        k0 = mats.k_rock_layers[q]
+        d = (J_of(layered(int(q), 1 + h)) - J_of(layered(int(q), 1 - h))) / (2 * h * k0)
+        fd.append({"param": f"k_layer_{int(q) + 1}", "adjoint": float(dk[q]), "central_difference": d})
+    for e in fd:
+        e["rel_diff"] = abs(e["adjoint"] - e["central_difference"]) / max(abs(e["central_difference"]), 1e-300)
+    res["fd_check"] = fd
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -199,6 +283,11 @@ def main():
                     help="timed steps of the 601x601x2002 FP32-vs-FP64 comparison (0: skip)")
     ap.add_argument("--large-err-steps", type=int, default=144,
                     help="steps after which the FP32 outlet error of that comparison is taken")
+    ap.add_argument("--adjoint-steps", type=int, default=8640,
+                    help="steps of the layered-borehole gradient (30 days at dt 300 s; 0: skip)")
+    ap.add_argument("--adjoint-checkpoint", type=int, default=64, help="checkpoint interval of the gradient")
+    ap.add_argument("--adjoint-fd", type=int, default=2,
+                    help="layer conductivities checked by central differences (besides the inlet)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -302,6 +391,11 @@ def main():
             del cache, u
         torch.cuda.empty_cache()
         out["large_fp32"] = bench_large(gw, args.large_steps, 2, args.large_err_steps)
+    if args.adjoint_steps > 0:
+        if "cache" in locals():
+            del cache, u
+        torch.cuda.empty_cache()
+        out["adjoint"] = bench_adjoint(gw, args.adjoint_steps, args.adjoint_checkpoint, args.adjoint_fd)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads, args.cpu_steps)
     print(json.dumps(out), flush=True)

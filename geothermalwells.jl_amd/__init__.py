@@ -6,6 +6,7 @@ The compute path is libgeowells_hip.so (HIP kernels + C ABI, include/geowells.h)
 this package is the host-side mirror of the reference's Julia interface.
 """
 from . import _abi
+from .adjoint import outlet_gradient
 from .cache import (Cache, CallbackSet, DiscreteCallback, SavedValues, create_batch_cache, create_cache,
                     get_simulation_callback)
 from .grids import compute_domain, create_adaptive_grid_1d, create_uniform_gridz_with_borehole_depths

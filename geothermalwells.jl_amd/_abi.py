@@ -85,6 +85,11 @@ SIGNATURES = {
                                   ctypes.c_double, ctypes.c_int32, ctypes.c_int32]),
     "gw_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int32, c_double_p]),
+    "gw_adjoint_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_double]),
+    "gw_adjoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                  ctypes.c_int32, c_double_p, c_double_p, c_double_p, ctypes.c_int32, c_double_p,
+                                  ctypes.c_void_p]),
 }
 
 _lib = None

@@ -36,7 +36,7 @@ def _run_all(cmds):
 
 
 KERNEL_UNITS = ["gw_kernels.hip", "gw_line_w64_f64.hip", "gw_line_w64_f32.hip", "gw_line_w32_f64.hip",
-                "gw_line_w32_f32.hip"]
+                "gw_line_w32_f32.hip", "gw_line_adj_w64.hip", "gw_line_adj_w32.hip", "gw_adjoint.hip"]
 
 
 def build(verbose=True):

@@ -109,14 +109,16 @@ struct Mat {
 // restricted to the boreholes `near` (all others are skipped by the
 // reference's radius test anyway).
 void point_material(double x, double y, double z, const std::vector<HostBh>& bhs,
-                    const std::vector<int>& near, const Mat& m, double& k, double& rc) {
+                    const std::vector<int>& near, const Mat& m, double& k, double& rc, bool* rock = nullptr) {
   k = m.rock_k(z);
   rc = m.rock_rc(z);
+  if (rock) *rock = true;
   for (int bi : near) {
     const HostBh& bh = bhs[bi];
     const double dx = x - bh.xc, dy = y - bh.yc;
     const double r_sq = dx * dx + dy * dy;
     if (!(r_sq > bh.r_backfill_sq)) {  // src/physics.jl:207-228
+      if (rock && z <= bh.h + 3) *rock = false;
       if (z <= bh.h) {
         if (r_sq < bh.r_inner * bh.r_inner) k = m.k_water;
         else if (r_sq < bh.r_and_t_inner_sq) k = m.k_ins;
@@ -166,6 +168,7 @@ struct Scen {
   std::vector<uint16_t> cls;      // nx*ny, index j + ny*i
   int ncls = 0;
   std::vector<double> kprof, rprof;  // ncls*nz
+  std::vector<uint8_t> rockprof;      // ncls*nz: 1 where the conductivity is the rock layer's
   std::vector<int32_t> zone;      // nz
   int nzone = 0;
   std::vector<int> zone_rep;      // representative level of each zone
@@ -195,6 +198,7 @@ struct CTab {
   RedChunk* d_rchunk = nullptr;
   RedChunk* d_agroup = nullptr;
   double* d_partial = nullptr;
+  std::vector<Dep> h_dep;           // host copy of the departure table (adjoint)
   int nmeans = 0;
   int nrchunk = 0;
   int nagroup = 0;
@@ -237,6 +241,7 @@ struct gw_ctx {
   int64_t cells_total = 0, cols_total = 0;
   // tile offsets of the line-length classes, per k-window family
   int64_t xcls[2][kLineClasses + 1] = {}, ycls[2][kLineClasses + 1] = {}, hcls[2][kLineClasses + 1] = {};
+  int32_t hslots[2][kLineClasses] = {};
   // k-window width (64 or 32) of each line operator (mode, direction): the
   // length default until the first gw_steps times both widths (autotune), or
   // the width the problem flags force
@@ -282,6 +287,7 @@ struct gw_ctx {
       v.xtiles[f] = d_xtiles[f]; v.ytiles[f] = d_ytiles[f]; v.htiles[f] = d_htiles[f];
       for (int q = 0; q <= kLineClasses; ++q) {
         v.xcls[f][q] = xcls[f][q]; v.ycls[f][q] = ycls[f][q]; v.hcls[f][q] = hcls[f][q];
+        if (q < kLineClasses) v.hslots[f][q] = hslots[f][q];
       }
     }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
@@ -321,28 +327,36 @@ void classify(Scen& s) {
   }
   // class 0: plain rock
   std::unordered_map<std::string, int> cmap;
-  s.kprof.clear(); s.rprof.clear();
+  s.kprof.clear(); s.rprof.clear(); s.rockprof.clear();
   std::vector<double> kp(nz), rp(nz);
+  std::vector<uint8_t> rk(nz);
   const std::vector<int> none;
   auto add_profile = [&](void) -> int {
     std::string key(reinterpret_cast<const char*>(kp.data()), nz * sizeof(double));
     key.append(reinterpret_cast<const char*>(rp.data()), nz * sizeof(double));
+    key.append(reinterpret_cast<const char*>(rk.data()), nz);
     auto it = cmap.find(key);
     if (it != cmap.end()) return it->second;
     const int id = (int)cmap.size();
     cmap.emplace(std::move(key), id);
     s.kprof.insert(s.kprof.end(), kp.begin(), kp.end());
     s.rprof.insert(s.rprof.end(), rp.begin(), rp.end());
+    s.rockprof.insert(s.rockprof.end(), rk.begin(), rk.end());
     return id;
   };
-  for (int k = 0; k < nz; ++k) point_material(0, 0, s.gz[k], s.bh, none, s.mat, kp[k], rp[k]);
+  auto material = [&](double x, double y, int k, const std::vector<int>& nr) {
+    bool r = true;
+    point_material(x, y, s.gz[k], s.bh, nr, s.mat, kp[k], rp[k], &r);
+    rk[k] = r ? 1 : 0;
+  };
+  for (int k = 0; k < nz; ++k) material(0, 0, k, none);
   add_profile();
   s.cls.assign((size_t)nx * ny, 0);
   for (int i = 0; i < nx; ++i)
     for (int j = 0; j < ny; ++j) {
       const auto& nr = near[(size_t)j + (size_t)ny * i];
       if (nr.empty()) continue;
-      for (int k = 0; k < nz; ++k) point_material(s.gx[i], s.gy[j], s.gz[k], s.bh, nr, s.mat, kp[k], rp[k]);
+      for (int k = 0; k < nz; ++k) material(s.gx[i], s.gy[j], k, nr);
       const int id = add_profile();
       if (id > 65535) throw std::runtime_error("more than 65536 material column classes");
       s.cls[(size_t)j + (size_t)ny * i] = (uint16_t)id;
@@ -426,16 +440,18 @@ void build_line_sets(gw_ctx& ctx, Scen& s, std::unordered_map<std::string, int>&
 }
 
 // Line tiles of one scenario and direction.  Each tile is a k window of lw
-// lanes (64 or 32) whose lanes meet at most kMaxSets coefficient sets.  Plain
-// tiles output all their lanes; halo tiles (halo = 3, for the fused z-step)
-// output lanes halo .. lw-1-halo, the others only feed the z-stencil of the
-// outputs.
-constexpr int kMaxSets = 3;
+// lanes (64 or 32) whose lanes meet at most 3 coefficient sets (plain tiles)
+// or 4 (halo tiles).  Plain tiles output all their lanes; halo tiles (halo =
+// 3, for the fused z-step) output lanes halo .. lw-1-halo, the others only
+// feed the z-stencil of the outputs.  A halo tile needs the sets of 7
+// consecutive levels for each output, and a layer boundary a few levels above
+// a borehole bottom (the casing's bottom level below it) puts 4 there.
 
 void line_tiles(const Scen& s, int si, int dir, int halo, int lw, std::vector<LineTile>& out) {
   const int nlines = dir == 0 ? s.ny : s.nx;
   const std::vector<int32_t>& ids = dir == 0 ? s.xset : s.yset;
   const int width = lw - 2 * halo;
+  const int kMaxSets = halo > 0 ? 4 : 3;
   for (int line = 0; line < nlines; ++line) {
     auto sid_at = [&](int k) {
       const int kc = std::min(std::max(k, 0), s.nz - 1);
@@ -460,11 +476,11 @@ void line_tiles(const Scen& s, int si, int dir, int halo, int lw, std::vector<Li
         ++n;
       }
       if (n == 0 || (int)sets.size() > kMaxSets)
-        throw std::runtime_error("more than 3 material zones within a 7-level z window");
+        throw std::runtime_error("more than 4 material zones within a 7-level z window");
       LineTile t{};
       t.scen = si; t.line = line; t.kw = k0 - halo; t.ko0 = k0; t.ko1 = k0 + n;
       t.nset = (int)sets.size();
-      for (int q = 0; q < 3; ++q) t.set[q] = q < t.nset ? sets[q] : -1;
+      for (int q = 0; q < 4; ++q) t.set[q] = q < t.nset ? sets[q] : -1;
       out.push_back(t);
       k0 += n;
     }
@@ -660,6 +676,7 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
     return fail(GW_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e.e));
   }
   t.h_bh = hb;
+  t.h_dep = dep;
   auto res = ctx.ctabs.emplace(c, t);
   out = &res.first->second;
   return GW_OK;
@@ -683,9 +700,9 @@ double stability_bound(double c3) {
 }
 
 // One advection sub-step (advection!, src/solver.jl:250-264) on phi.
-int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau) {
+int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* jacc = nullptr) {
   DevView v = ctx.view(t);
-  int rc = hip_status(launch_pipe_reduce(v, ctx.prec, phi, ctx.stream), "pipe_reduce");
+  int rc = hip_status(launch_pipe_reduce(v, ctx.prec, phi, ctx.stream, jacc), "pipe_reduce");
   if (rc) return rc;
   if (ctx.host_inlet) {
     std::vector<double> tout(ctx.nbh_total), inl(ctx.nbh_total);
@@ -897,6 +914,11 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       gather(xb[f], xt[f], ctx->xcls[f]);
       gather(yb[f], yt[f], ctx->ycls[f]);
       gather(hb[f], ht[f], ctx->hcls[f]);
+      for (int q = 0; q < kLineClasses; ++q) {
+        int m = 1;
+        for (const LineTile& t : hb[f][q]) m = std::max(m, (int)t.nset);
+        ctx->hslots[f][q] = m;
+      }
     }
     ctx->d_sc = ctx->up(sd);
     ctx->d_cell_prefix = ctx->up(cellp);
@@ -996,14 +1018,15 @@ int fused_zy(gw_ctx& ctx, const CTab* tab, const void* in, void* out, double dt)
 
 // One step's advection/x part (in place on `cur`) and its closing y kernel
 // from `cur` into `next` (YZY, or the plain implicit y when `last`).
-int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end, double dt, bool last) {
+int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end, double dt, bool last,
+               double* jacc = nullptr) {
   const double c = dt / 2;
   DevView v = ctx.view(tab);
   int rc = sub_step_advect(ctx, tab, cur, t_end);
   if (rc) return rc;
   rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx.stream), "x");
   if (rc) return rc;
-  rc = sub_step_advect(ctx, tab, cur, t_end + c);
+  rc = sub_step_advect(ctx, tab, cur, t_end + c, jacc);
   if (rc) return rc;
   if (last)
     return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream), "implicit y");
@@ -1277,6 +1300,387 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint of the outlet objective (one scenario, FP64).
+//
+//   J = (1/nsteps) sum_n sum_b w_b T_out_b(n),   T_out_b(n) the outlet of
+//   step n's second advection sub-step,
+//
+// differentiated with respect to the rock conductivity (per layer, or k_rock)
+// and the inlet temperatures, by reverse sweeps over checkpointed segments:
+// the forward run (the fused pipeline) keeps the state every `seg` steps;
+// each segment is recomputed with the separate operators, keeping u_n, the
+// z-step result z_n and the implicit-x result e_n of every step, and then
+// swept backwards.  The state adjoint runs in mu = lambda / W (see AdjGeo).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Device buffers of one call, freed on every exit path.
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <class T>
+  T* alloc(size_t n) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) throw std::bad_alloc();
+    p.push_back(q);
+    return static_cast<T*>(q);
+  }
+  template <class T>
+  T* up(const std::vector<T>& h) {
+    T* d = alloc<T>(h.size());
+    if (!h.empty() && hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      throw HipFail{hipErrorInvalidValue};
+    return d;
+  }
+};
+
+// Control lengths and inverse face spacings of a grid (the denominators of
+// line_coefficients: h_l = (dp + dm) / 2 with mirrored ends).
+void grid_h(const std::vector<double>& g, std::vector<double>& h, std::vector<double>& inv) {
+  const int N = (int)g.size();
+  h.assign(N, 0.0);
+  inv.assign(N, 0.0);
+  for (int l = 0; l < N; ++l) {
+    const double dp = (l == N - 1) ? (g[l] - g[l - 1]) : (g[l + 1] - g[l]);
+    const double dm = (l == 0) ? dp : (g[l] - g[l - 1]);
+    h[l] = (dp + dm) * 0.5;
+    if (l + 1 < N) inv[l] = 1.0 / (g[l + 1] - g[l]);
+  }
+}
+
+// Transposed advection tables of one step length (departure table of c).
+AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
+  const Scen& s = ctx.sc[0];
+  const int nl = s.countz;
+  std::vector<AdjBh> abh(ctx.nbh_total);
+  std::vector<int32_t> off, toff;
+  std::vector<AdjEnt> ent, tent;
+  std::vector<RedChunk> ch;
+  int poff = 0, soff = 0;
+  constexpr int kChunk = 64;
+  for (int b = 0; b < ctx.nbh_total; ++b) {
+    const BhDev& B = tab.h_bh[b];
+    AdjBh& A = abh[b];
+    auto dep = [&](int k, int kind) { return tab.h_dep[(size_t)B.dep_off + 2 * (size_t)k + kind]; };
+    // gather lists: lambda_in[k'] += w lambda_out[k] for KEEP (k' = k) and OWN (k' = kl, kr)
+    for (int kind = 0; kind < 2; ++kind) {
+      std::vector<std::vector<AdjEnt>> lists(nl);
+      for (int k = 0; k < nl; ++k) {
+        const Dep e = dep(k, kind);
+        if (e.type == DEP_KEEP) {
+          lists[k].push_back(AdjEnt{k, 0, 1.0});
+        } else if (e.type == DEP_OWN) {
+          lists[e.kl].push_back(AdjEnt{k, 0, 1.0 - e.a});
+          lists[e.kr].push_back(AdjEnt{k, 0, e.a});
+        }
+      }
+      (kind == 0 ? A.csr_in : A.csr_an) = (int32_t)off.size();
+      for (int k = 0; k < nl; ++k) {
+        off.push_back((int32_t)ent.size());
+        ent.insert(ent.end(), lists[k].begin(), lists[k].end());
+      }
+      off.push_back((int32_t)ent.size());
+    }
+    // level ranges of the turnaround (inner) and inlet (annulus) departures
+    int t0 = nl, t1 = -1, i0 = nl, i1 = -1;
+    for (int k = 0; k < nl; ++k) {
+      if (dep(k, 0).type == DEP_TURN) { t0 = std::min(t0, k); t1 = std::max(t1, k); }
+      if (dep(k, 1).type == DEP_INLET) { i0 = std::min(i0, k); i1 = std::max(i1, k); }
+    }
+    A.t0 = t1 >= 0 ? t0 : 0;
+    A.nt = t1 >= 0 ? t1 - t0 + 1 : 0;
+    A.i0 = i1 >= 0 ? i0 : 0;
+    A.ni = i1 >= 0 ? i1 - i0 + 1 : 0;
+    // turnaround transpose over the annulus mean levels klo .. klo+nlev-1
+    std::vector<std::vector<AdjEnt>> tl(std::max(B.nlev, 0));
+    for (int k = 0; k < nl; ++k) {
+      const Dep e = dep(k, 0);
+      if (e.type != DEP_TURN) continue;
+      tl[e.kl - B.klo].push_back(AdjEnt{k, 0, 1.0 - e.a});
+      tl[e.kr - B.klo].push_back(AdjEnt{k, 0, e.a});
+    }
+    A.turn_off = (int32_t)toff.size();
+    for (int q = 0; q < B.nlev; ++q) {
+      toff.push_back((int32_t)tent.size());
+      tent.insert(tent.end(), tl[q].begin(), tl[q].end());
+    }
+    toff.push_back((int32_t)tent.size());
+    A.soff = soff;
+    soff += std::max(B.nlev, 0);
+    // reduction chunks: inner columns over the turnaround levels, annulus
+    // columns over the inlet levels
+    A.red0 = (int32_t)ch.size();
+    if (A.nt > 0)
+      for (int c0 = 0; c0 < B.n_in; c0 += kChunk) {
+        ch.push_back(RedChunk{b, 0, B.col_in0 + c0, std::min(kChunk, B.n_in - c0), poff, 0});
+        poff += A.nt;
+      }
+    A.nred_in = (int32_t)ch.size() - A.red0;
+    if (A.ni > 0)
+      for (int c0 = 0; c0 < B.n_an; c0 += kChunk) {
+        ch.push_back(RedChunk{b, 1, B.col_an0 + c0, std::min(kChunk, B.n_an - c0), poff, 0});
+        poff += A.ni;
+      }
+    A.nred_an = (int32_t)ch.size() - A.red0 - A.nred_in;
+  }
+  AdjPipe ap{};
+  ap.bh = db.up(abh);
+  ap.off = db.up(off);
+  ap.ent = db.up(ent);
+  ap.toff = db.up(toff);
+  ap.tent = db.up(tent);
+  ap.chunk = db.up(ch);
+  ap.nchunk = (int32_t)ch.size();
+  ap.partial = db.alloc<double>((size_t)poff);
+  ap.sbuf = db.alloc<double>((size_t)soff);
+  ap.gtin = db.up(std::vector<double>(ctx.nbh_total, 0.0));
+  return ap;
+}
+
+// n fused steps from t on phi (as gw_steps, without graphs), adding every
+// step's second-sub-step outlet to jacc.
+int run_fused(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, int n, double* jacc) {
+  void* B = ctx.buf2;
+  int rc = fused_zy(ctx, tab, phi, B, dt);
+  if (rc) return rc;
+  for (int k = 1; k < n; ++k) {
+    const bool odd = (k & 1) != 0;
+    if ((rc = fused_rest(ctx, tab, odd ? B : phi, odd ? phi : B, t + k * dt, dt, false, jacc))) return rc;
+  }
+  return fused_rest(ctx, tab, (n & 1) ? B : phi, phi, t + n * dt, dt, true, jacc);
+}
+
+// One step with the separate operators from u, keeping z = R(dt A_z) u and
+// e = (I - cA_x)^-1 b; u_next receives the step's result.
+int adj_recompute(gw_ctx& ctx, const CTab* tab, const double* u, double* z, double* e, double* u_next,
+                  double* wb, double t_end, double dt) {
+  const double c = dt / 2;
+  const DevView v = ctx.view(tab);
+  const size_t bytes = (size_t)ctx.cells_total * sizeof(double);
+  int rc = hip_status(hipMemcpyAsync(z, u, bytes, hipMemcpyDeviceToDevice, ctx.stream), "copy");
+  if (!rc) rc = hip_status(launch_zstep(v, GW_F64, z, dt, ctx.c3, ctx.stream), "zstep");
+  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), wb, z, c, 0.0, 0.0,
+                                       ctx.stream), "explicit y");
+  if (!rc) rc = sub_step_advect(ctx, tab, wb, t_end);
+  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 0, ctx.lw(LINE_IMPLICIT, 0), e, wb, c, 0.0, 0.0,
+                                       ctx.stream), "implicit x");
+  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 0, ctx.lw(LINE_EXPLICIT, 0), wb, e, c, 0.0, 0.0,
+                                       ctx.stream), "explicit x");
+  if (!rc) rc = sub_step_advect(ctx, tab, wb, t_end + c);
+  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), u_next, wb, c, 0.0,
+                                       0.0, ctx.stream), "implicit y");
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void* x, double dt) {
+  if (!ctx || !out || !in || op < 0 || op > 5) return fail(GW_EINVAL, "bad argument");
+  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
+  if (ctx->sc.size() != 1 || ctx->prec != GW_F64 || !ctx->fused_ok)
+    return fail(GW_EINVAL, "the adjoint operators take one FP64 scenario");
+  if ((op == 1 || op == 2 || op == 4) && !x) return fail(GW_EINVAL, "this operator needs the forward field x");
+  gw_ctx& C = *ctx;
+  const Scen& s = C.sc[0];
+  try {
+    CTab* tab = nullptr;
+    int rc = prepare_steps(C, in, dt, tab);
+    if (rc) return rc;
+    const double c = dt / 2;
+    DevBufs db;
+    std::vector<double> hx, hy, hz, ix, iy, iz;
+    grid_h(s.gx, hx, ix);
+    grid_h(s.gy, hy, iy);
+    grid_h(s.gz, hz, iz);
+    std::vector<double> rock(s.rockprof.begin(), s.rockprof.end());
+    AdjGeo ag{};
+    ag.hx = db.up(hx); ag.hy = db.up(hy); ag.hz = db.up(hz);
+    ag.ix = db.up(ix); ag.iy = db.up(iy); ag.iz = db.up(iz);
+    ag.rho = db.up(s.rprof);
+    ag.rock = db.up(rock);
+    ag.gx_part = db.up(std::vector<double>((size_t)s.ny * s.nz, 0.0));
+    ag.gy_part = db.up(std::vector<double>((size_t)s.nx * s.nz, 0.0));
+    const DevView v = C.view(tab);
+    double* o = static_cast<double*>(out);
+    const double* i = static_cast<const double*>(in);
+    const double* xf = static_cast<const double*>(x);
+    const size_t bytes = (size_t)C.cells_total * sizeof(double);
+    switch (op) {
+      case 0: {
+        const AdjPipe ap = build_adj_pipe(C, *tab, db);
+        rc = hip_status(hipMemcpyAsync(o, i, bytes, hipMemcpyDeviceToDevice, C.stream), "copy");
+        if (!rc) rc = hip_status(launch_adj_pipe(v, ag, ap, o, nullptr, C.stream), "adjoint advection");
+        break;
+      }
+      case 1:
+      case 2: {
+        const int mode = op == 1 ? ADJ_Y : ADJ_X;
+        const int lw = mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1) : C.lw(LINE_IMPLICIT_EXPLICIT, 0);
+        rc = hip_status(lw == 32 ? line_adj_w32(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream)
+                                 : line_adj_w64(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream),
+                        "adjoint line");
+        break;
+      }
+      case 3:
+        rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 1, C.lw(LINE_EXPLICIT, 1), o, i, c, 0.0, 0.0,
+                                    C.stream), "explicit y");
+        break;
+      case 4:
+        rc = hip_status(launch_adj_z(v, ag, o, i, xf, dt, C.c3, C.stream), "adjoint z");
+        break;
+      default:
+        rc = hip_status(launch_mul_w(v, ag, o, i, C.stream), "W mu");
+        break;
+    }
+    if (!rc) rc = hip_status(hipStreamSynchronize(C.stream), "sync");
+    return rc;
+  } catch (const std::bad_alloc&) {
+    return fail(GW_ENOMEM, "cannot allocate the adjoint tables");
+  } catch (const HipFail& e) {
+    return fail(GW_EHIP, std::string("adjoint set-up: ") + hipGetErrorString(e.e));
+  }
+}
+
+int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nsteps, int32_t seg,
+               const double* w_bh, double* J, double* dJ_dk, int32_t n_dk, double* dJ_dTin, void* dJ_du0) {
+  if (!ctx || !phi0 || !J || !dJ_dk || !dJ_dTin) return fail(GW_EINVAL, "null argument");
+  if (!(dt > 0) || nsteps < 1 || seg < 1) return fail(GW_EINVAL, "dt must be positive, nsteps and seg >= 1");
+  if (ctx->sc.size() != 1) return fail(GW_EINVAL, "the adjoint takes one scenario");
+  if (ctx->prec != GW_F64) return fail(GW_EINVAL, "the adjoint runs in FP64");
+  if (!ctx->fused_ok) return fail(GW_ESTATE, "too many column classes for the adjoint kernels");
+  const Scen& s = ctx->sc[0];
+  if (s.inlet.kind != GW_INLET_CONSTANT && s.inlet.kind != GW_INLET_PER_WELL)
+    return fail(GW_EINVAL, "the adjoint supports constant and per-well inlets");
+  const int nlay = s.mat.depth.empty() ? 1 : (int)s.mat.depth.size();
+  if (n_dk != nlay) return fail(GW_EINVAL, "dJ_dk must have one entry per rock layer (1 when homogeneous)");
+  gw_ctx& C = *ctx;
+  try {
+    CTab* tab = nullptr;
+    int rc = prepare_steps(C, phi0, dt, tab);
+    if (rc) return rc;
+    const double c = dt / 2;
+    const size_t cells = (size_t)C.cells_total, bytes = cells * sizeof(double);
+    const int nbh = C.nbh_total;
+    DevBufs db;
+    // geometry and material tables
+    std::vector<double> hx, hy, hz, ix, iy, iz;
+    grid_h(s.gx, hx, ix);
+    grid_h(s.gy, hy, iy);
+    grid_h(s.gz, hz, iz);
+    std::vector<double> rock(s.rockprof.begin(), s.rockprof.end());
+    AdjGeo ag{};
+    ag.hx = db.up(hx); ag.hy = db.up(hy); ag.hz = db.up(hz);
+    ag.ix = db.up(ix); ag.iy = db.up(iy); ag.iz = db.up(iz);
+    ag.rho = db.up(s.rprof);
+    ag.rock = db.up(rock);
+    ag.gx_part = db.up(std::vector<double>((size_t)s.ny * s.nz, 0.0));
+    ag.gy_part = db.up(std::vector<double>((size_t)s.nx * s.nz, 0.0));
+    const AdjPipe ap = build_adj_pipe(C, *tab, db);
+    // storage: checkpoints, one segment of u / z / e, two adjoint fields, a work field
+    const int nseg = (nsteps + seg - 1) / seg;
+    const int sl = std::min(seg, nsteps);
+    std::vector<double*> ck(nseg), U(sl + 1), Z(sl), E(sl);
+    for (auto& p : ck) p = db.alloc<double>(cells);
+    for (auto& p : U) p = db.alloc<double>(cells);
+    for (auto& p : Z) p = db.alloc<double>(cells);
+    for (auto& p : E) p = db.alloc<double>(cells);
+    double* MU0 = db.alloc<double>(cells);
+    double* MU1 = db.alloc<double>(cells);
+    double* WB = db.alloc<double>(cells);
+    double* jacc = db.up(std::vector<double>(nbh, 0.0));
+    // forward with checkpoints
+    if ((rc = hip_status(hipMemcpyAsync(U[0], phi0, bytes, hipMemcpyDeviceToDevice, C.stream), "copy"))) return rc;
+    for (int sg = 0; sg < nseg; ++sg) {
+      const int n = std::min(seg, nsteps - sg * seg);
+      if ((rc = hip_status(hipMemcpyAsync(ck[sg], U[0], bytes, hipMemcpyDeviceToDevice, C.stream), "copy")))
+        return rc;
+      if ((rc = run_fused(C, tab, U[0], t0 + (double)sg * seg * dt, dt, n, jacc))) return rc;
+    }
+    std::vector<double> w(nbh), hj(nbh);
+    for (int b = 0; b < nbh; ++b) w[b] = w_bh ? w_bh[b] : 1.0 / nbh;
+    if ((rc = hip_status(hipMemcpyAsync(hj.data(), jacc, sizeof(double) * nbh, hipMemcpyDeviceToHost, C.stream),
+                         "copy")) ||
+        (rc = hip_status(hipStreamSynchronize(C.stream), "sync")))
+      return rc;
+    double jv = 0.0;
+    for (int b = 0; b < nbh; ++b) jv += w[b] * hj[b];
+    *J = jv / nsteps;
+    // objective seed per borehole: w_b / (nsteps n_in n_out) on the inner outlet levels
+    std::vector<double> seedh(nbh);
+    for (int b = 0; b < nbh; ++b)
+      seedh[b] = w[b] / ((double)nsteps * (double)tab->h_bh[b].n_in * (double)tab->h_bh[b].n_outlev);
+    const double* seed = db.up(seedh);
+    // backward
+    const DevView v = C.view(tab);
+    if ((rc = hip_status(hipMemsetAsync(MU0, 0, bytes, C.stream), "memset"))) return rc;
+    auto adj_line = [&](int mode, double* out, const double* in, const double* x1) {
+      const int lw = mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1) : C.lw(LINE_IMPLICIT_EXPLICIT, 0);
+      return hip_status(lw == 32 ? line_adj_w32(v, ag, mode, out, in, x1, nullptr, c, 0.0, 0.0, 0.0, C.stream)
+                                 : line_adj_w64(v, ag, mode, out, in, x1, nullptr, c, 0.0, 0.0, 0.0, C.stream),
+                        "adjoint line");
+    };
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+      const int n = std::min(seg, nsteps - sg * seg);
+      const int base = sg * seg;
+      if ((rc = hip_status(hipMemcpyAsync(U[0], ck[sg], bytes, hipMemcpyDeviceToDevice, C.stream), "copy")))
+        return rc;
+      for (int m = 0; m < n; ++m)
+        if ((rc = adj_recompute(C, tab, U[m], Z[m], E[m], U[m + 1], WB, t0 + (double)(base + m + 1) * dt, dt)))
+          return rc;
+      for (int m = n - 1; m >= 0; --m) {
+        // mu_{n+1} in MU0 -> mu_n in MU0
+        if ((rc = adj_line(ADJ_Y, MU1, MU0, U[m + 1]))) return rc;
+        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, MU1, seed, C.stream), "adjoint advection"))) return rc;
+        if ((rc = adj_line(ADJ_X, MU0, MU1, E[m]))) return rc;
+        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, MU0, nullptr, C.stream), "adjoint advection"))) return rc;
+        if ((rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 1, C.lw(LINE_EXPLICIT, 1), MU1, MU0, c, 0.0, 0.0,
+                                         C.stream), "explicit y")))
+          return rc;
+        if ((rc = hip_status(launch_grad_stream(v, ag, 1, MU0, nullptr, Z[m], c, C.stream), "gradient y"))) return rc;
+        if ((rc = hip_status(launch_adj_z(v, ag, MU0, MU1, U[m], dt, C.c3, C.stream), "adjoint z"))) return rc;
+      }
+    }
+    if (dJ_du0 && (rc = hip_status(launch_mul_w(v, ag, static_cast<double*>(dJ_du0), MU0, C.stream), "W mu")))
+      return rc;
+    // gradients to the host: per level over the lines in order, then per layer
+    std::vector<double> gxp((size_t)s.ny * s.nz), gyp((size_t)s.nx * s.nz), gt(nbh);
+    if ((rc = hip_status(hipMemcpyAsync(gxp.data(), ag.gx_part, sizeof(double) * gxp.size(), hipMemcpyDeviceToHost,
+                                        C.stream), "copy")) ||
+        (rc = hip_status(hipMemcpyAsync(gyp.data(), ag.gy_part, sizeof(double) * gyp.size(), hipMemcpyDeviceToHost,
+                                        C.stream), "copy")) ||
+        (rc = hip_status(hipMemcpyAsync(gt.data(), ap.gtin, sizeof(double) * nbh, hipMemcpyDeviceToHost, C.stream),
+                         "copy")) ||
+        (rc = hip_status(hipStreamSynchronize(C.stream), "sync")))
+      return rc;
+    for (int q = 0; q < nlay; ++q) dJ_dk[q] = 0.0;
+    for (int k = 0; k < s.nz; ++k) {
+      double tot = 0.0;
+      for (int j = 0; j < s.ny; ++j) tot += gxp[(size_t)j * s.nz + k];
+      for (int i = 0; i < s.nx; ++i) tot += gyp[(size_t)i * s.nz + k];
+      int lay = 0;
+      if (!s.mat.depth.empty()) {
+        lay = nlay - 1;
+        for (int q = 0; q < nlay; ++q)
+          if (s.gz[k] <= s.mat.depth[q]) { lay = q; break; }
+      }
+      dJ_dk[lay] += tot;
+    }
+    for (int b = 0; b < nbh; ++b) dJ_dTin[b] = gt[b];
+    return GW_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(GW_ENOMEM, "cannot allocate the adjoint's checkpoints and work fields");
+  } catch (const HipFail& e) {
+    return fail(GW_EHIP, std::string("adjoint set-up: ") + hipGetErrorString(e.e));
+  }
 }
 
 }  // extern "C"

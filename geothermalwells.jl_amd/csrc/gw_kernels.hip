@@ -179,10 +179,12 @@ __global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __res
     });
 }
 
-// Pass 2: one block per borehole.
-__global__ __launch_bounds__(256) void k_pipe_final(DevView v) {
+// Pass 2: one block per borehole; jacc (may be NULL) accumulates the outlet
+// of every launch per borehole (the block owns its slot).
+__global__ __launch_bounds__(256) void k_pipe_final(DevView v, double* jacc) {
   __shared__ double red[256];
   pipe_final(v, blockIdx.x, red);
+  if (jacc && threadIdx.x == 0) jacc[blockIdx.x] += v.tout[blockIdx.x];
 }
 
 // One block per advection group (up to kAdvGroup pipe columns of one borehole
@@ -308,7 +310,7 @@ hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, vo
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st) {
+hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st, double* jacc) {
   if (v.nbh == 0) return hipSuccess;
   if (prec == 0)
     hipLaunchKernelGGL((k_pipe_partial<double>), dim3(v.nrchunk), dim3(256), 0, st, v, (const double*)phi);
@@ -316,7 +318,7 @@ hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipSt
     hipLaunchKernelGGL((k_pipe_partial<float>), dim3(v.nrchunk), dim3(256), 0, st, v, (const float*)phi);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pipe_final, dim3(v.nbh), dim3(256), 0, st, v);
+  hipLaunchKernelGGL(k_pipe_final, dim3(v.nbh), dim3(256), 0, st, v, jacc);
   return hipGetLastError();
 }
 

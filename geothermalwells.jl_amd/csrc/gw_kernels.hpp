@@ -30,6 +30,7 @@ struct DevView {
   const LineTile* ytiles[2];    // y-line tiles
   const LineTile* htiles[2];    // y-line windows with a 3-lane k halo (fused z-step)
   int64_t xcls[2][kLineClasses + 1], ycls[2][kLineClasses + 1], hcls[2][kLineClasses + 1];
+  int32_t hslots[2][kLineClasses];  // most coefficient sets of a halo tile of the class (LDS slots)
   const uint16_t* cls;
   const int32_t* zone;
   const double* zcoef;          // pairs {ea, eb}
@@ -57,6 +58,84 @@ struct DevView {
 
 enum LineMode { LINE_EXPLICIT = 0, LINE_IMPLICIT = 1, LINE_IMPLICIT_EXPLICIT = 2, LINE_ZY = 3, LINE_YZY = 4 };
 
+// ---------------------------------------------------------------------------
+// Adjoint (one scenario, FP64).  The 1-D operators are A = D^-1 K with K
+// symmetric and D = h * rho_c (h the control length of the direction), so
+// with the cell weight W = hx hy hz rho_c every transposed diffusion operator
+// is W op W^-1: the adjoint runs the forward line operators on mu = lambda / W
+// (reverse order), and only the advection needs a transposed kernel.
+// ---------------------------------------------------------------------------
+struct AdjGeo {
+  const double* hx;      // [nx] control lengths (h_l = (dp + dm) / 2, mirrored ends)
+  const double* hy;      // [ny]
+  const double* hz;      // [nz]
+  const double* ix;      // [nx] 1 / (g[l+1] - g[l]); 0 at the last point
+  const double* iy;      // [ny]
+  const double* iz;      // [nz]
+  const double* rho;     // [ncls][nz] volumetric heat capacity per column class
+  const double* rock;    // [ncls][nz] 1 where the conductivity is the rock layer's, else 0
+  double* gx_part;       // [ny][nz] conductivity-gradient sums of the x-line kernels (per cell level)
+  double* gy_part;       // [nx][nz] ... of the y-line kernels (y faces and z faces)
+};
+
+// Adjoint line modes (mu in, mu out; gradients added to the part buffers):
+//   ADJ_Y    mu <- S_y mu,                 grad c G_y(mu_out, x1)
+//   ADJ_X    mu <- S_x E_x mu,             grad c G_x(mu_in, x1) + c G_x(mu_out, x1)
+// with G_d(mu, x) = sum over faces of direction d of -(A_f / D_f) (mu2 - mu1)(x2 - x1), half to each
+// rock cell of the face.
+enum AdjLineMode { ADJ_Y = 0, ADJ_X = 1 };
+
+hipError_t line_adj_w64(const DevView& v, const AdjGeo& ag, int mode, double* out, const double* in,
+                        const double* x1, const double* x2, double c, double a1, double a2, double a3,
+                        hipStream_t st);
+hipError_t line_adj_w32(const DevView& v, const AdjGeo& ag, int mode, double* out, const double* in,
+                        const double* x1, const double* x2, double c, double a1, double a2, double a3,
+                        hipStream_t st);
+
+// Streaming adjoint kernels (no line held in registers): per tile of levels
+// (lanes) one pass along the line.
+//   grad_stream: grad += scale * G_dir(mu1 + mu2, x)   (mu2 may be NULL)
+//   adj_z:       out = R(dt A_z) mu (the transposed z-step in mu), and the
+//                z-face gradient of its three Horner stages with the forward
+//                stage values recomputed from u
+hipError_t launch_grad_stream(const DevView& v, const AdjGeo& ag, int dir, const double* mu1, const double* mu2,
+                              const double* x, double scale, hipStream_t st);
+hipError_t launch_adj_z(const DevView& v, const AdjGeo& ag, double* out, const double* mu, const double* u,
+                        double dt, double c3, hipStream_t st);
+
+// Transposed advection of one sub-step, in mu: per borehole, gather lists of
+// the departure table (CSR over the destination level), the turnaround
+// transpose (CSR over annulus levels), chunked reductions over the pipe
+// columns.  seed (may be NULL): per-borehole objective weight added to the
+// inner-pipe outlet levels (the sub-step whose outlet enters the objective).
+struct AdjEnt {
+  int32_t k, pad;
+  double w;
+};
+struct AdjBh {
+  int32_t csr_in, csr_an;   // offsets into adj_off of the inner / annulus level CSR (countz + 1 entries)
+  int32_t turn_off;         // offset into turn_off_tab (nlev + 1 entries) of the turnaround transpose
+  int32_t t0, nt;           // inner levels reduced for the turnaround (t0 .. t0+nt-1)
+  int32_t i0, ni;           // annulus levels reduced for the inlet (i0 .. i0+ni-1)
+  int32_t red0, nred_in, nred_an, soff;  // chunks; offset of the annulus-level sums in sbuf
+};
+struct AdjPipe {
+  const AdjBh* bh;
+  const int32_t* off;       // CSR offsets
+  const AdjEnt* ent;        // CSR entries (source level, weight)
+  const int32_t* toff;      // turnaround CSR offsets (per annulus level)
+  const AdjEnt* tent;       // turnaround entries (inner level, weight)
+  const RedChunk* chunk;    // reduction chunks (part 0: inner/turn levels, 1: annulus/inlet levels)
+  int32_t nchunk;
+  double* partial;          // chunk partial sums
+  double* sbuf;             // per-borehole annulus-level adjoint of the turnaround means
+  double* gtin;             // [nbh] accumulated d objective / d inlet temperature
+};
+hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap, double* mu, const double* seed,
+                           hipStream_t st);
+// lam = W mu over the whole field (the state adjoint from mu)
+hipError_t launch_mul_w(const DevView& v, const AdjGeo& ag, double* lam, const double* mu, hipStream_t st);
+
 // Default k-window width (64 or 32 lanes) for lines of up to nmax elements.
 int line_window_default(int nmax);
 // The line kernels of one k-window family and state precision (a1..a3: the
@@ -77,7 +156,8 @@ hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double
 hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, void* out, const void* in,
                        double c, double dt_z, double c3, hipStream_t st);
 // Outlet sums, annulus level means and inlet values of one sub-step.
-hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st);
+// jacc (may be NULL): per-borehole outlet accumulator, += T_outlet of this sub-step.
+hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st, double* jacc = nullptr);
 // Semi-Lagrangian update of every pipe column, in place.
 hipError_t launch_pipe_advect(const DevView& v, int prec, void* phi, hipStream_t st);
 

@@ -68,11 +68,11 @@ struct Dep {
 
 // A line tile: the 64 lanes k = kw .. kw+63 (clamped to the grid) of x-line
 // j (or y-line i) = `line` in scenario `scen`; lanes with ko0 <= k < ko1 are
-// written.  Its lanes use the nset (<= 3) coefficient sets in `set`.
+// written.  Its lanes use the nset coefficient sets in `set` (at most 3 in
+// plain tiles, 4 in the z-halo tiles of the fused z-step; unused entries -1).
 struct LineTile {
   int32_t scen, line, kw, ko0, ko1, nset;
-  int32_t set[3];
-  int32_t pad;
+  int32_t set[4];
 };
 
 // Coefficient set of one 1-D line (length N) for one half-step length c,

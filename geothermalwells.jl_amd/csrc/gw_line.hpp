@@ -49,7 +49,8 @@ struct LineCfg {
   static_assert(LW_ == 64 || LW_ == 32, "k-window of a full or half wave");
 };
 
-constexpr int kSlots = 3;      // coefficient sets per window
+constexpr int kSlots = 3;      // coefficient sets per plain window (LDS slots)
+constexpr int kSlotsHalo = 4;  // most sets per z-halo window
 constexpr int kZcMax = 16;     // column classes staged for the fused z-step
 
 template <int MODE> struct ModeTraits {
@@ -62,11 +63,17 @@ template <int MODE> struct ModeTraits {
   static constexpr int A_BD = 0, A_FW = 1, A_BU = 2, A_EA = impl ? 3 : 0, A_EB = A_EA + 1;
 };
 
+// LDS of a tile, in doubles: carries and edges, the z coefficients and column
+// classes of the halo modes, then `slots` coefficient-set slots (last, so the
+// slot count only changes the launch's LDS size).
 template <class Cfg, int MODE>
-constexpr size_t line_lds_doubles() {
+constexpr size_t line_lds_fixed() {
   using MT = ModeTraits<MODE>;
-  return 2 * Cfg::G * Cfg::LW + kSlots * MT::NA * Cfg::NMAX +
-         (MT::halo ? kZcMax * Cfg::LW * 2 + (Cfg::NMAX + 1) / 2 : 0);
+  return 2 * Cfg::G * Cfg::LW + (MT::halo ? kZcMax * Cfg::LW * 2 + (Cfg::NMAX + 1) / 2 : 0);
+}
+template <class Cfg, int MODE>
+constexpr size_t line_lds_doubles(int slots) {
+  return line_lds_fixed<Cfg, MODE>() + (size_t)slots * ModeTraits<MODE>::NA * Cfg::NMAX;
 }
 
 GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
@@ -89,9 +96,9 @@ __global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW] carries / edges
   double* shB = shA + G * LW;                          // [G][LW]
-  double* shC = shB + G * LW;                          // [kSlots][MT::NA][NM] coefficient arrays
-  double* shZ = shC + kSlots * MT::NA * NM;            // [kZcMax][LW][2]: z coefficients
+  double* shZ = shB + G * LW;                          // [kZcMax][LW][2]: z coefficients (halo modes)
   int* shK = reinterpret_cast<int*>(shZ + kZcMax * LW * 2);  // [NM] column class per line element
+  double* shC = shA + line_lds_fixed<Cfg, MODE>();     // [nset][MT::NA][NM] coefficient arrays
 
   constexpr int FAM = LW == 32 ? 1 : 0;
   const LineTile* tiles = MT::halo ? v.htiles[FAM] : (dir == 0 ? v.xtiles[FAM] : v.ytiles[FAM]);
@@ -119,9 +126,9 @@ __global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS
   // coefficient sets of the window's lanes, identity rows beyond N
   const int32_t* sids = v.setid + (dir == 0 ? S_.o_xset : S_.o_yset) + (int64_t)T.line * S_.nzone;
   const int sid = sids[v.zone[S_.o_zone + kc]];
-  const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
+  const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : (sid == T.set[3] ? 3 : 0));
   for (int q = 0; q < T.nset; ++q) {
-    const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : T.set[2]);
+    const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : (q == 2 ? T.set[2] : T.set[3]));
     const double* g = v.coef + v.set_off[sq];
     double* dst = shC + q * MT::NA * NM;
     for (int l = tid; l < NM; l += Cfg::THREADS) {
@@ -368,23 +375,27 @@ __global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS
 
 template <class Cfg, class TS, int MODE>
 static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
-                        double a3, int64_t tile0, int64_t ntiles, hipStream_t st) {
-  const size_t sh = line_lds_doubles<Cfg, MODE>() * sizeof(double);
+                        double a3, int64_t tile0, int64_t ntiles, int slots, hipStream_t st) {
+  const size_t sh = line_lds_doubles<Cfg, MODE>(slots) * sizeof(double);
   hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
                      c, a1, a2, a3, tile0, ntiles);
 }
 
 template <class Cfg, class TS>
 static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const TS* in, double c, double a1,
-                           double a2, double a3, int64_t tile0, int64_t ntiles, hipStream_t st) {
+                           double a2, double a3, int64_t tile0, int64_t ntiles, int hslots, hipStream_t st) {
   switch (mode) {
-    case LINE_EXPLICIT: line_launch<Cfg, TS, LINE_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
-    case LINE_IMPLICIT: line_launch<Cfg, TS, LINE_IMPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
-    case LINE_IMPLICIT_EXPLICIT:
-      line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st);
+    case LINE_EXPLICIT:
+      line_launch<Cfg, TS, LINE_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
       break;
-    case LINE_ZY: line_launch<Cfg, TS, LINE_ZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
-    case LINE_YZY: line_launch<Cfg, TS, LINE_YZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, st); break;
+    case LINE_IMPLICIT:
+      line_launch<Cfg, TS, LINE_IMPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+      break;
+    case LINE_IMPLICIT_EXPLICIT:
+      line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+      break;
+    case LINE_ZY: line_launch<Cfg, TS, LINE_ZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
+    case LINE_YZY: line_launch<Cfg, TS, LINE_YZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -450,18 +461,19 @@ hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const vo
   for (int q = 0; q < kLineClasses; ++q) {
     const int64_t t0 = cls[q], n = cls[q + 1] - cls[q];
     if (n == 0) continue;
+    const int hs = halo ? v.hslots[f][q] : kSlots;
     hipError_t e;
     switch (q) {
-      case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
-      default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, st); break;
+      case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
     }
     if (e != hipSuccess) return e;
   }

@@ -1,0 +1,366 @@
+// gw_line_adj.hpp -- adjoint line kernels (FP64), on the tiles of the line
+// kernel (gw_line.hpp).  The state adjoint runs in mu = lambda / W, where the
+// transposed diffusion operators are the forward ones (see AdjGeo), so the
+// solve and the explicit half-step below repeat the forward kernel's
+// arithmetic on mu.  Each mode also accumulates the gradient of the
+// objective with respect to the rock conductivity: per face of the operator's
+// direction the term -(A_f / D_f) (mu2 - mu1)(x2 - x1), x the forward field
+// the operator acted on, half to each rock cell of the face, summed per
+// (line, level) over the line (one lane per level) and added to the line's
+// slot of gx_part / gy_part by one lane (no atomics; fixed order).
+#pragma once
+
+#include "gw_line.hpp"
+
+namespace gw {
+
+template <int MODE> struct AdjTraits {
+  static constexpr bool halo = false;
+  static constexpr bool impl = true;
+  static constexpr bool expl = MODE == ADJ_X;
+  static constexpr int NA = (impl ? 3 : 0) + (expl ? 2 : 0);
+  static constexpr int A_BD = 0, A_FW = 1, A_BU = 2, A_EA = impl ? 3 : 0, A_EB = A_EA + 1;
+};
+
+template <class Cfg, int MODE>
+constexpr size_t line_adj_lds_doubles() {
+  using AT = AdjTraits<MODE>;
+  // carries/edges, coefficient sets, z coefficients (halo), rock flags,
+  // inverse face spacing and h along the line, class per element
+  return 2 * Cfg::G * Cfg::LW + kSlots * AT::NA * Cfg::NMAX + (AT::halo ? kZcMax * Cfg::LW * 2 : 0) +
+         kZcMax * Cfg::LW + 2 * Cfg::NMAX + (Cfg::NMAX + 1) / 2;
+}
+
+template <class Cfg, int MODE>
+__global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag, int dir, double* out,
+                                                           const double* in, const double* x1, const double* x2,
+                                                           double c, double a1, double a2, double a3,
+                                                           int64_t tile0, int64_t ntiles) {
+  using AT = AdjTraits<MODE>;
+  constexpr int S = Cfg::S, NM = Cfg::NMAX, LW = Cfg::LW, H = Cfg::H, G = Cfg::G, KC = 4;
+  static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW]
+  double* shB = shA + G * LW;                          // [G][LW]
+  double* shC = shB + G * LW;                          // [kSlots][NA][NM]
+  double* shZ = shC + kSlots * AT::NA * NM;            // [kZcMax][LW][2] (halo modes)
+  double* shR = shZ + (AT::halo ? kZcMax * LW * 2 : 0);  // [kZcMax][LW] rock flags
+  double* shI = shR + kZcMax * LW;                     // [NM] 1 / face spacing along the line
+  double* shH = shI + NM;                              // [NM] control length along the line
+  int* shK = reinterpret_cast<int*>(shH + NM);         // [NM] column class per element
+
+  constexpr int FAM = LW == 32 ? 1 : 0;
+  const LineTile* tiles = AT::halo ? v.htiles[FAM] : (dir == 0 ? v.xtiles[FAM] : v.ytiles[FAM]);
+  const LineTile T = tiles[tile0 + xcd_remap(blockIdx.x, ntiles)];
+  const ScenDev S_ = v.sc[T.scen];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int kk = lane % LW;
+  const int hh = lane / LW;
+  const int seg = wv * H + hh;
+  const int N = dir == 0 ? S_.nx : S_.ny;
+  const int nz = S_.nz;
+  const int k = T.kw + kk;
+  const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
+  const int kmin = T.kw < 0 ? 0 : T.kw;
+  const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(double) +
+                      (unsigned)(hh * S) * (unsigned)(dir == 0 ? nz * S_.ny : nz) * (unsigned)sizeof(double);
+  const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
+  const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
+
+  // coefficient sets (identity rows beyond N), as the forward kernel
+  const int32_t* sids = v.setid + (dir == 0 ? S_.o_xset : S_.o_yset) + (int64_t)T.line * S_.nzone;
+  const int sid = sids[v.zone[S_.o_zone + kc]];
+  const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
+  for (int q = 0; q < T.nset; ++q) {
+    const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : T.set[2]);
+    const double* g = v.coef + v.set_off[sq];
+    double* dst = shC + q * AT::NA * NM;
+    for (int l = tid; l < NM; l += Cfg::THREADS) {
+      const bool ok = l < N;
+      if (AT::impl) {
+        dst[AT::A_BD * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
+        dst[AT::A_FW * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
+        dst[AT::A_BU * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
+      }
+      if (AT::expl) {
+        dst[AT::A_EA * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
+        dst[AT::A_EB * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
+      }
+    }
+  }
+  // classes along the line, rock flags and z coefficients of the window's levels
+  const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
+  for (int l = tid; l < NM; l += Cfg::THREADS) {
+    const int lc = l < N ? l : N - 1;
+    const int64_t col = dir == 0 ? (int64_t)lc * S_.ny + T.line : (int64_t)T.line * S_.ny + lc;
+    shK[l] = v.cls[S_.o_cls + col];
+    const double* gi = dir == 0 ? ag.ix : ag.iy;
+    const double* gh = dir == 0 ? ag.hx : ag.hy;
+    shI[l] = l + 1 < N ? gi[l] : 0.0;
+    shH[l] = gh[lc];
+  }
+  for (int q = wv; q < ncls; q += Cfg::NW)
+    if (hh == 0) shR[q * LW + kk] = ag.rock[(int64_t)q * nz + kc];
+
+  double d[S];
+  const int l0 = seg * S;
+  const int64_t ustride = uniform64(stride);
+  const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
+  {
+    const double* pin = in + useg;
+#pragma unroll
+    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? *lane_ptr(pin + (int64_t)r * ustride, kb) : 0.0;
+  }
+  __syncthreads();
+
+  const double* cs = shC + slot * AT::NA * NM;
+  const double* bd = cs + AT::A_BD * NM;
+  const double* fw = cs + AT::A_FW * NM;
+  const double* bu = cs + AT::A_BU * NM;
+  const double* ea = cs + AT::A_EA * NM;
+  const double* eb = cs + AT::A_EB * NM;
+  const bool kin = k >= 0 && k < nz;
+  // face factor of this lane's level: the other two control lengths
+  const double hline = dir == 0 ? ag.hy[T.line] : ag.hx[T.line];
+  const double phi_line = kin ? hline * ag.hz[kc] : 0.0;
+  double acc = 0.0;
+
+  // sum over this segment's faces along the line of (mu2-mu1)(x2-x1) times
+  // the half rock weights of the two cells, times -c * phi_line
+  auto grad_line = [&](const double* x) __attribute__((always_inline)) {
+    shA[seg * LW + kk] = d[0];
+    __syncthreads();
+    const double mu_edge = (l0 + S < N) ? shA[(seg + 1) * LW + kk] : 0.0;
+    __syncthreads();
+    const double* px = x + useg;
+    double xr = (l0 < N) ? *lane_ptr(px, kb) : 0.0;
+    double xn = (l0 + 1 < N) ? *lane_ptr(px + ustride, kb) : 0.0;
+    double gsum = 0.0;
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+      const int l = l0 + r;
+      // the next element's load is issued one element ahead; the barrier keeps
+      // the compiler from hoisting all of them (which would spill the line)
+      const double xnn = (l + 2 < N && r + 2 <= S) ? *lane_ptr(px + (int64_t)(r + 2) * ustride, kb) : 0.0;
+      const double mn = (r + 1 < S) ? d[r + 1] : mu_edge;
+      if (l + 1 < N) {
+        const double wgt = shR[shK[l] * LW + kk] + shR[shK[l + 1] * LW + kk];
+        gsum += shI[l] * (mn - d[r]) * (xn - xr) * wgt;
+      }
+      xr = xn;
+      xn = xnn;
+      asm volatile("" ::: "memory");
+    }
+    acc += -0.5 * c * phi_line * gsum;
+  };
+
+  // the solve and the explicit step keep the forward kernel's chunked loops
+  // (the next chunk's LDS operands fetched while the current one computes)
+  auto thomas = [&]() __attribute__((always_inline)) {
+    double y = 0.0, P = 1.0, wn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = fw[l0 + q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = fw[l0 + r0 + KC + q];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        y = d[r0 + q] - wc[q] * y;
+        P = -wc[q] * P;
+        d[r0 + q] = y;
+      }
+    }
+    shA[seg * LW + kk] = y;
+    shB[seg * LW + kk] = P;
+    __syncthreads();
+    double C = 0.0;
+    for (int g = 0; g < seg; ++g) C = shA[g * LW + kk] + shB[g * LW + kk] * C;
+    __syncthreads();
+    P = 1.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = fw[l0 + q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = fw[l0 + r0 + KC + q];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        P = -wc[q] * P;
+        d[r0 + q] = d[r0 + q] + P * C;
+      }
+    }
+    double x = 0.0, Q = 1.0, bn[KC], un[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      bn[q] = bd[l0 + S - 1 - q];
+      un[q] = bu[l0 + S - 1 - q];
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double bc[KC], uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { bc[q] = bn[q]; uc[q] = un[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) {
+          bn[q] = bd[l0 + S - 1 - (r0 + KC + q)];
+          un[q] = bu[l0 + S - 1 - (r0 + KC + q)];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        x = bc[q] * d[r] - uc[q] * x;
+        Q = -uc[q] * Q;
+        d[r] = x;
+      }
+    }
+    shA[seg * LW + kk] = x;
+    shB[seg * LW + kk] = Q;
+    __syncthreads();
+    double X = 0.0;
+    for (int g = G - 1; g > seg; --g) X = shA[g * LW + kk] + shB[g * LW + kk] * X;
+    __syncthreads();
+    Q = 1.0;
+#pragma unroll
+    for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) uc[q] = un[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - (r0 + KC + q)];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        Q = -uc[q] * Q;
+        d[r] = d[r] + Q * X;
+      }
+    }
+  };
+
+  auto explicit_step = [&]() __attribute__((always_inline)) {
+    shA[seg * LW + kk] = d[0];
+    shB[seg * LW + kk] = d[S - 1];
+    __syncthreads();
+    double prev = (seg > 0) ? shB[(seg - 1) * LW + kk] : d[0];
+    const double edge = (l0 + S < N) ? shA[(seg + 1) * LW + kk] : d[S - 1];
+    __syncthreads();
+    double an[KC], bn2[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) { an[q] = ea[l0 + q]; bn2[q] = eb[l0 + q]; }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double ac[KC], bc2[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { ac[q] = an[q]; bc2[q] = bn2[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) { an[q] = ea[l0 + r0 + KC + q]; bn2[q] = eb[l0 + r0 + KC + q]; }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = r0 + q;
+        const double xv = d[r];
+        const double xp = (r < S - 1) ? d[r + 1] : edge;
+        d[r] = xv + c * (ac[q] * (xp - xv) - bc2[q] * (xv - prev));
+        prev = xv;
+      }
+    }
+  };
+
+  if (MODE == ADJ_Y) {
+    thomas();
+    grad_line(x1);
+  } else {
+    grad_line(x1);
+    explicit_step();
+    thomas();
+    grad_line(x1);
+  }
+
+  // per-(line, level) gradient: segments summed in order by segment 0
+  shA[seg * LW + kk] = acc;
+  __syncthreads();
+  if (seg == 0 && k >= T.ko0 && k < T.ko1) {
+    double tot = 0.0;
+    for (int g = 0; g < G; ++g) tot += shA[g * LW + kk];
+    double* part = dir == 0 ? ag.gx_part : ag.gy_part;
+    part[(int64_t)T.line * nz + k] += tot;
+  }
+
+  if (k >= T.ko0 && k < T.ko1) {
+    double* pout = out + useg;
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+      if (l0 + r < N) *lane_ptr(pout + (int64_t)r * ustride, kb) = d[r];
+  }
+}
+
+template <class Cfg, int MODE>
+static void line_adj_launch(const DevView& v, const AdjGeo& ag, int dir, double* out, const double* in,
+                            const double* x1, const double* x2, double c, double a1, double a2, double a3,
+                            int64_t tile0, int64_t ntiles, hipStream_t st) {
+  const size_t sh = line_adj_lds_doubles<Cfg, MODE>() * sizeof(double);
+  hipLaunchKernelGGL((k_line_adj<Cfg, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, ag, dir, out,
+                     in, x1, x2, c, a1, a2, a3, tile0, ntiles);
+}
+
+template <class Cfg>
+static hipError_t line_adj_cfg(const DevView& v, const AdjGeo& ag, int mode, double* out, const double* in,
+                               const double* x1, const double* x2, double c, double a1, double a2, double a3,
+                               int64_t t0, int64_t n, hipStream_t st) {
+  switch (mode) {
+    case ADJ_Y: line_adj_launch<Cfg, ADJ_Y>(v, ag, 1, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+    case ADJ_X: line_adj_launch<Cfg, ADJ_X>(v, ag, 0, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// One launch per non-empty line-length class of the mode's tiles.
+template <int LW>
+hipError_t line_adj_family(const DevView& v, const AdjGeo& ag, int mode, double* out, const double* in,
+                           const double* x1, const double* x2, double c, double a1, double a2, double a3,
+                           hipStream_t st) {
+  using F = LineFamily<LW>;
+  const int f = LW == 32 ? 1 : 0;
+  const int64_t* cls = mode == ADJ_X ? v.xcls[f] : v.ycls[f];
+  for (int q = 0; q < kLineClasses; ++q) {
+    const int64_t t0 = cls[q], n = cls[q + 1] - cls[q];
+    if (n == 0) continue;
+    hipError_t e;
+    switch (q) {
+      case 0: e = line_adj_cfg<typename F::C0>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 1: e = line_adj_cfg<typename F::C1>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 2: e = line_adj_cfg<typename F::C2>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 3: e = line_adj_cfg<typename F::C3>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 4: e = line_adj_cfg<typename F::C4>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 5: e = line_adj_cfg<typename F::C5>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 6: e = line_adj_cfg<typename F::C6>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 7: e = line_adj_cfg<typename F::C7>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      case 8: e = line_adj_cfg<typename F::C8>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+      default: e = line_adj_cfg<typename F::C9>(v, ag, mode, out, in, x1, x2, c, a1, a2, a3, t0, n, st); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace gw

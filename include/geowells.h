@@ -160,6 +160,34 @@ int gw_advect(gw_ctx *ctx, void *phi, double c, double t);
  * Modes 3 and 4 are y-direction only and need out != in. */
 int gw_line_op(gw_ctx *ctx, void *out, const void *in, double c, double dt_z, int32_t dir, int32_t mode);
 
+/* Gradient of the outlet objective
+ *   J = (1/nsteps) sum_n sum_b w_b T_out_b(n),
+ * T_out_b(n) borehole b's outlet in step n's second advection sub-step
+ * (src/solver.jl:253-257, 458), with respect to the rock conductivity of every
+ * layer (k_rock when homogeneous: StratifiedMaterialProperties /
+ * HomogenousMaterialProperties, src/physics.jl:81-94, 159-170) and every
+ * borehole's inlet temperature, from phi0 (read only) over nsteps steps of dt
+ * from t0 -- the derivative a caller of the reference obtains by
+ * differentiating solve(...) through its parameters.  Reverse sweeps over
+ * segments of seg steps (one checkpoint each, recomputed).  One scenario,
+ * GW_F64, constant or per-well inlets.  w_bh NULL: w_b = 1/n_bh.  dJ_dk has
+ * n_dk = max(1, n_layers) entries; dJ_dTin has n_bh (for a constant inlet
+ * dJ/dT is their sum).  dJ_du0 (device, may be NULL) receives dJ/dphi0.
+ * Allocates its working fields; synchronous. */
+int gw_adjoint(gw_ctx *ctx, const void *phi0, double t0, double dt, int32_t nsteps, int32_t seg,
+               const double *w_bh, double *J, double *dJ_dk, int32_t n_dk, double *dJ_dTin, void *dJ_du0);
+
+/* The pieces of gw_adjoint's reverse step, for callers that compose their
+ * own adjoint (and for testing), in mu = lambda / W with W = hx hy hz rho_c
+ * per cell (h the control lengths of the grid): op 0 transposed advection
+ * sub-step of length dt/2; 1 transposed implicit y (x: the forward result,
+ * for the conductivity gradient, discarded here); 2 transposed implicit +
+ * explicit x (x: the forward implicit-x result); 3 transposed explicit y;
+ * 4 transposed z-step R(dt A_z) (x: the forward field it acted on);
+ * 5 lambda = W mu.  One FP64 scenario; synchronous; no counterpart in the
+ * reference (it has no adjoint). */
+int gw_adjoint_op(gw_ctx *ctx, int32_t op, void *out, const void *in, const void *x, double dt);
+
 /* Introspection (cache fields the reference exposes). */
 int64_t gw_state_elems(gw_ctx *ctx);
 int64_t gw_scenario_offset(gw_ctx *ctx, int32_t scen);

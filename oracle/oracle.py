@@ -70,6 +70,9 @@ def lib():
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_dmax.restype = ctypes.c_double
         L.or_dmax.argtypes = [ctypes.c_void_p]
+        L.or_objective_dual.restype = ctypes.c_int
+        L.or_objective_dual.argtypes = [ctypes.POINTER(OrScenario), _dp, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.or_set_threads.argtypes = [ctypes.c_int]
         L.or_max_threads.restype = ctypes.c_int
         _lib = L
@@ -187,6 +190,22 @@ class Oracle:
         u = np.array(u, dtype=np.float64, copy=True, order="C")
         self._chk(lib().or_rock2_step(self.h, _p(u), dt, c3))
         return u
+
+    def objective_dual(self, u, t0, dt, nsteps, c3, w_bh=None):
+        """Forward-mode (dual-number) derivative of J = (1/nsteps) sum_n sum_b
+        w_b T_out_b(n) (the outlet of each step's second sub-step; w_b = 1/nbh
+        by default) with respect to the layer conductivities (or k_rock) and
+        the inlet temperature(s).  Returns (J, dJ)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        w = np.ascontiguousarray(np.full(self.nbh, 1.0 / self.nbh) if w_bh is None else w_bh, dtype=np.float64)
+        J = ctypes.c_double()
+        dJ = np.zeros(32)
+        n = ctypes.c_int()
+        rc = lib().or_objective_dual(ctypes.byref(self._s), _p(u), t0, dt, nsteps, c3, _p(w), ctypes.byref(J),
+                                     _p(dJ), ctypes.byref(n))
+        if rc != 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return J.value, dJ[:n.value].copy()
 
     def run(self, u, t0, dt, nsteps, c3):
         """Returns (u_final, outlet history [nsteps, nbh], seconds)."""

@@ -49,32 +49,85 @@ static std::vector<Borehole> make_bhs(int nbh, const double* a) {
   return v;
 }
 
+}  // extern "C"
+
+template <class T>
+static void fill_problem(Problem<T>* p, const or_scenario* s) {
+  p->gx.assign(s->gx, s->gx + s->nx);
+  p->gy.assign(s->gy, s->gy + s->ny);
+  p->gz.assign(s->gz, s->gz + s->nz);
+  p->bhs = make_bhs(s->nbh, s->bh);
+  auto& m = p->mat;
+  if (s->n_layers > 0) {
+    m.k_rock_layers.assign(s->k_layers, s->k_layers + s->n_layers);
+    m.rho_c_rock_layers.assign(s->rc_layers, s->rc_layers + s->n_layers);
+    m.layer_depths.assign(s->depths, s->depths + s->n_layers);
+  }
+  m.k_rock = s->mat[0]; m.rho_c_rock = s->mat[1];
+  m.k_water = s->mat[2]; m.rho_c_water = s->mat[3];
+  m.k_steel = s->mat[4]; m.rho_c_steel = s->mat[5];
+  m.k_insulating = s->mat[6]; m.rho_c_insulating = s->mat[7];
+  m.k_backfill = s->mat[8]; m.rho_c_backfill = s->mat[9];
+  p->inlet.kind = s->inlet_kind;
+  p->inlet.value = s->inlet_value;
+  if (s->inlet_kind == 2) p->inlet.per_well.assign(s->inlet_per_well, s->inlet_per_well + s->nbh);
+  gwo::build_cache(*p);
+}
+
+extern "C" {
+
 void* or_create(const or_scenario* s) {
   try {
     auto* p = new Problem<double>();
-    p->gx.assign(s->gx, s->gx + s->nx);
-    p->gy.assign(s->gy, s->gy + s->ny);
-    p->gz.assign(s->gz, s->gz + s->nz);
-    p->bhs = make_bhs(s->nbh, s->bh);
-    auto& m = p->mat;
-    if (s->n_layers > 0) {
-      m.k_rock_layers.assign(s->k_layers, s->k_layers + s->n_layers);
-      m.rho_c_rock_layers.assign(s->rc_layers, s->rc_layers + s->n_layers);
-      m.layer_depths.assign(s->depths, s->depths + s->n_layers);
-    }
-    m.k_rock = s->mat[0]; m.rho_c_rock = s->mat[1];
-    m.k_water = s->mat[2]; m.rho_c_water = s->mat[3];
-    m.k_steel = s->mat[4]; m.rho_c_steel = s->mat[5];
-    m.k_insulating = s->mat[6]; m.rho_c_insulating = s->mat[7];
-    m.k_backfill = s->mat[8]; m.rho_c_backfill = s->mat[9];
-    p->inlet.kind = s->inlet_kind;
-    p->inlet.value = s->inlet_value;
-    if (s->inlet_kind == 2) p->inlet.per_well.assign(s->inlet_per_well, s->inlet_per_well + s->nbh);
-    gwo::build_cache(*p);
+    fill_problem(p, s);
     return p;
   } catch (const std::exception& e) {
     g_err = e.what();
     return nullptr;
+  }
+}
+
+// Forward-mode derivative (dual numbers, the role ForwardDiff plays for the
+// reference) of the outlet objective
+//   J = (1/nsteps) sum_n sum_b w_b T_out_b(n),
+// T_out_b(n) the outlet of step n's second advection sub-step, from phi0 at t0.
+// Parameters, in order: the rock conductivity of every layer (k_rock when the
+// materials are homogeneous), then the inlet: its constant temperature
+// (constant inlet) or one temperature per well (per-well table); a
+// heat-exchanger inlet contributes no parameter.  n_params receives their
+// count, dJ their derivatives (at most 32).
+int or_objective_dual(const or_scenario* s, const double* phi0, double t0, double dt, int nsteps, double c3,
+                      const double* w_bh, double* J, double* dJ, int* n_params) {
+  constexpr int N = 32;
+  using D = gwo::Dual<N>;
+  try {
+    const int nk = s->n_layers > 0 ? s->n_layers : 1;
+    const int nt = s->inlet_kind == 0 ? 1 : (s->inlet_kind == 2 ? s->nbh : 0);
+    if (nk + nt > N) { g_err = "more than 32 parameters"; return -1; }
+    Problem<D> p;
+    fill_problem(&p, s);
+    if (s->n_layers > 0)
+      for (int l = 0; l < nk; ++l) p.mat.k_rock_layers[l].d[l] = 1.0;
+    else
+      p.mat.k_rock.d[0] = 1.0;
+    if (s->inlet_kind == 0) p.inlet.value.d[nk] = 1.0;
+    if (s->inlet_kind == 2)
+      for (int b = 0; b < s->nbh; ++b) p.inlet.per_well[b].d[nk + b] = 1.0;
+    const gwo::Rock2Coef coef = gwo::rock2_coef_from_c3(c3);
+    std::vector<D> phi(phi0, phi0 + p.cells()), temp(p.cells()), s0, s1, s2;
+    D acc(0.0);
+    for (int n = 0; n < nsteps; ++n) {
+      gwo::full_step(p, phi.data(), temp.data(), t0 + (n + 1) * dt, dt, coef, s0, s1, s2);
+      for (int b = 0; b < s->nbh; ++b) acc += D(w_bh[b]) * p.T_outlet[b];
+    }
+    acc = acc / D((double)nsteps);
+    *J = acc.v;
+    for (int q = 0; q < nk + nt; ++q) dJ[q] = acc.d[q];
+    *n_params = nk + nt;
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
   }
 }
 

@@ -48,6 +48,7 @@ CASES = {
     "layered": dict(layered=True),
     "two_bh_hx": dict(nbh=2, inlet="hx"),
     "two_bh_perwell_ins": dict(nbh=2, inlet="perwell", zins=20.0),
+    "layer_above_bottom": dict(layered=True, depths=(20.0, 57.0, 65.0)),
 }
 
 

@@ -232,3 +232,53 @@ def test_rock2_polynomial_and_order():
     expect = u + z1 + z2 / 2 + z3 / 16
     got = orc.rock2_step(u, dt, 1.0 / 16.0)
     assert np.abs(got - expect).max() <= 1e-12 * np.abs(u).max()
+
+
+def _with_layers(g, w, k_layers):
+    m = w["materials"]
+    return dict(w, materials=g.StratifiedMaterialProperties(k_layers, m.rho_c_rock_layers, m.layer_depths, m.k_water,
+                                                            m.rho_c_water, m.k_steel, m.rho_c_steel, m.k_insulating,
+                                                            m.rho_c_insulating, m.k_backfill, m.rho_c_backfill))
+
+
+@pytest.mark.parametrize("inlet", ["constant", "perwell"])
+def test_dual_objective_matches_central_differences(inlet):
+    """The forward-mode (dual-number) derivative of the mean outlet, which the
+    GPU adjoint is checked against, equals central differences of the
+    oracle's own objective (h = 1e-6 relative) for every layer conductivity
+    and inlet temperature; its J equals the mean of the plain run's outlets."""
+    g = H.gw()
+    nbh = 2 if inlet == "perwell" else 1
+    w = H.small_case(layered=True, nbh=nbh, inlet=inlet)
+    orc = H.oracle_for(w)
+    u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+    n, dt, c3, h = 4, 60.0, 1.0 / 16.0, 1e-6
+    J, dJ = orc.objective_dual(u0, 0.0, dt, n, c3)
+    _, hist, _ = orc.run(u0, 0.0, dt, n, c3)
+    assert abs(J - hist.mean()) <= 1e-13 * abs(J)
+
+    def J_of(ww):
+        o = H.oracle_for(ww)
+        return o.objective_dual(u0, 0.0, dt, n, c3)[0]
+
+    kl = list(w["materials"].k_rock_layers)
+    for q in range(len(kl)):
+        up, dn = list(kl), list(kl)
+        up[q] *= 1 + h
+        dn[q] *= 1 - h
+        fd = (J_of(_with_layers(g, w, up)) - J_of(_with_layers(g, w, dn))) / (2 * h * kl[q])
+        assert abs(dJ[q] - fd) <= 1e-6 * abs(dJ).max(), (q, dJ[q], fd)
+    if inlet == "constant":
+        T0 = w["inlet_model"].T_const
+        fd = (J_of(dict(w, inlet_model=g.ConstantInlet(T0 * (1 + h)))) -
+              J_of(dict(w, inlet_model=g.ConstantInlet(T0 * (1 - h))))) / (2 * h * T0)
+        assert abs(dJ[len(kl)] - fd) <= 1e-6 * abs(fd), (dJ[len(kl)], fd)
+    else:
+        T = list(w["inlet_model"].values)
+        for b in range(nbh):
+            up, dn = list(T), list(T)
+            up[b] *= 1 + h
+            dn[b] *= 1 - h
+            fd = (J_of(dict(w, inlet_model=g.PerWellInlet(up))) -
+                  J_of(dict(w, inlet_model=g.PerWellInlet(dn)))) / (2 * h * T[b])
+            assert abs(dJ[len(kl) + b] - fd) <= 1e-6 * abs(fd), (b, dJ[len(kl) + b], fd)
