@@ -7,7 +7,8 @@
 
 namespace gw {
 
-constexpr int kStreamWaves = 4;   // waves per streaming block; each walks a quarter of the line
+constexpr int kStreamWaves = 4;   // waves per streaming block
+constexpr int kWalk = 16;         // elements per wave of k_adj_yz (fully unrolled: no loop back-edge)
 constexpr int kZcMaxStream = 16;  // column classes staged by k_adj_z (the adjoint requires at most this many)
 
 GW_DEV double cell_w(const AdjGeo& ag, const DevView& v, const ScenDev& S, int i, int j, int k) {
@@ -29,64 +30,31 @@ GW_DEV void add_line_part(double acc, double* part, const LineTile& T, int nz, d
   }
 }
 
-// grad += scale * G_dir(mu1 + mu2, x) on the 64-lane tiles of the direction:
-// per face (l-1, l) along the line, -(A_f / D_f)(dmu)(dx) with A_f / D_f =
-// h_other(line) hz(k) / (g[l] - g[l-1]), half to each rock cell.  Wave w
-// walks elements [l_a, l_b) of the line, starting from element l_a - 1.
-__global__ __launch_bounds__(64 * kStreamWaves) void k_grad_stream(DevView v, AdjGeo ag, int dir, const double* mu1,
-                                                                  const double* mu2, const double* x, double scale) {
-  __shared__ double sh[64 * kStreamWaves];
-  const LineTile T = (dir == 0 ? v.xtiles[0] : v.ytiles[0])[blockIdx.x];
-  const ScenDev S = v.sc[T.scen];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int N = dir == 0 ? S.nx : S.ny, nz = S.nz;
-  const int k = T.kw + lane;
-  const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
-  const bool kin = k >= 0 && k < nz;
-  const int per = (N + kStreamWaves - 1) / kStreamWaves;
-  const int la = w * per, lb = min(N, la + per);
-  auto cell = [&](int l) {
-    const int64_t col = dir == 0 ? (int64_t)l * S.ny + T.line : (int64_t)T.line * S.ny + l;
-    return S.off + col * nz + kc;
-  };
-  auto rock_at = [&](int l) {
-    const int64_t col = dir == 0 ? (int64_t)l * S.ny + T.line : (int64_t)T.line * S.ny + l;
-    return ag.rock[(int64_t)v.cls[S.o_cls + col] * nz + kc];
-  };
-  const double* gi = dir == 0 ? ag.ix : ag.iy;
-  // faces (l-1, l) for l in [max(la, 1), lb): the wave starts from element
-  // la - 1 (or 0), so the face at a wave boundary is counted exactly once
-  double g = 0.0;
-  if (la < lb) {
-    int l = la > 0 ? la - 1 : 0;
-    const int64_t c0 = cell(l);
-    double mp = mu1[c0] + (mu2 ? mu2[c0] : 0.0), xp = x[c0], rp = rock_at(l);
-    for (l = l + 1; l < lb; ++l) {
-      const int64_t c1 = cell(l);
-      const double m = mu1[c1] + (mu2 ? mu2[c1] : 0.0), xv = x[c1], r = rock_at(l);
-      g += gi[l - 1] * (m - mp) * (xv - xp) * (rp + r);
-      mp = m; xp = xv; rp = r;
-    }
-  }
-  const double hline = dir == 0 ? ag.hy[T.line] : ag.hx[T.line];
-  const double acc = kin ? -0.5 * scale * hline * ag.hz[kc] * g : 0.0;
-  add_line_part(acc, dir == 0 ? ag.gx_part : ag.gy_part, T, nz, sh);
-}
-
-// Transposed z-step R(dt A_z) in mu on the 64-lane halo tiles of the y-lines
-// (3-level halo, outputs 3..60), one line element (column (i, j)) at a time,
-// with the z-face gradient of the three Horner stages:
-//   forward  w1 = u + a1 A u,  w2 = u + a2 A w1         (recomputed from u)
-//   adjoint  m2 = a3 A m,  m1 = a2 A m2,  out = m + m2 + m1 + a1 A m1
-//   grad     sum_s a_s G_z(m_s, w_{s-1}) over (m, w2), (m2, w1), (m1, u)
-__global__ __launch_bounds__(64 * kStreamWaves) void k_adj_z(DevView v, AdjGeo ag, double* out, const double* mu,
-                                                            const double* u, double a1, double a2, double a3) {
+// Transposed (explicit y then z-step) in mu on the 64-lane halo tiles of the
+// y-lines (3-level halo, outputs 3..60), with both gradients.  The line is
+// split over ag.yparts blocks of kStreamWaves waves; each wave walks kWalk
+// consecutive elements (columns (i, j)), loads two to three elements ahead,
+// and keeps only a few values per lane.  Per element:
+//   EY:       m = mu + c (ea (mu_{j+1} - mu) - eb (mu - mu_{j-1}))   (explicit y in mu)
+//             grad_y += face (j-1, j): (mu_j - mu_{j-1})(Z_j - Z_{j-1}), Z = R(dt A_z) u
+//   forward   w1 = u + a1 A u,  w2 = u + a2 A w1,  Z = u + a3 A w2   (from u, in the wave)
+//   adjoint   m2 = a3 A m,  m1 = a2 A m2,  out = m + m2 + m1 + a1 A m1
+//   grad_z    sum_s a_s G_z(m_s, w_{s-1}) over (m, w2), (m2, w1), (m1, u)
+// Without EY, m = mu (the transposed z-step alone).  Gradients go to slot
+// (part, line) of gy_part.
+template <bool EY>
+__global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo ag, double* out, const double* mu,
+                                                             const double* u, double c, double a1, double a2,
+                                                             double a3) {
   __shared__ double shZ[kZcMaxStream * 64 * 2];
   __shared__ double shR[kZcMaxStream * 64];
   __shared__ double sh[64 * kStreamWaves];
-  const LineTile T = v.htiles[0][blockIdx.x];
+  const int parts = ag.yparts;
+  const int64_t tb = blockIdx.x / parts;
+  const int p = (int)(blockIdx.x - tb * parts);
+  const LineTile T = v.htiles[0][tb];
   const ScenDev S = v.sc[T.scen];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int N = S.ny, nz = S.nz, i = T.line;
   const int k = T.kw + lane;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
@@ -100,29 +68,119 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_z(DevView v, AdjGeo a
   __syncthreads();
   const double izk = (k >= 0 && k < nz - 1) ? ag.iz[k] : 0.0;
   const double hxi = ag.hx[i];
+  const bool kin = k >= 0 && k < nz;
   const bool outl = k >= T.ko0 && k < T.ko1;
-  const int per = (N + kStreamWaves - 1) / kStreamWaves;
-  const int la = w * per, lb = min(N, la + per);
-  double acc = 0.0;
-  for (int j = la; j < lb; ++j) {
-    const int cl = __builtin_amdgcn_readfirstlane(v.cls[S.o_cls + (int64_t)i * S.ny + j]);
-    const double ea = shZ[(cl * 64 + lane) * 2], eb = shZ[(cl * 64 + lane) * 2 + 1];
-    auto Aop = [&](double q) { return ea * (wave_shl1(q) - q) - eb * (q - wave_shr1(q)); };
-    auto face = [&](double mm, double xx) { return (wave_shl1(mm) - mm) * (wave_shl1(xx) - xx); };
-    const int64_t cidx = S.off + ((int64_t)i * S.ny + j) * nz + kc;
-    const double uu = u[cidx], m = mu[cidx];
-    const double w1 = uu + a1 * Aop(uu);
-    const double w2 = uu + a2 * Aop(w1);
-    const double m2 = a3 * Aop(m);
-    const double m1 = a2 * Aop(m2);
-    const double o = m + m2 + m1 + a1 * Aop(m1);
-    const double fs = a3 * face(m, w2) + a2 * face(m2, w1) + a1 * face(m1, uu);
-    const double vup = -hxi * ag.hy[j] * izk * fs;
-    const double vdn = wave_shr1(vup);
-    acc += 0.5 * shR[cl * 64 + lane] * (vup + vdn);
-    if (outl) out[cidx] = o;
+  // explicit-y coefficients of this lane's level along line i: byte offsets
+  // of the lane's set arrays from the coefficient table (uniform base + a
+  // constant 32-bit lane offset per load)
+  unsigned ofa = 0, ofb = 0;
+  if (EY) {
+    const int sid = v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]];
+    const int64_t so = v.set_off[sid];
+    ofa = (unsigned)((so + SET_EA * (int64_t)N) * (int64_t)sizeof(double));
+    ofb = (unsigned)((so + SET_EB * (int64_t)N) * (int64_t)sizeof(double));
   }
-  add_line_part(acc, ag.gy_part, T, nz, sh);
+  // each wave walks kWalk elements (fewer at the end of the line)
+  const int la = min(N, (p * kStreamWaves + w) * kWalk), lb = min(N, la + kWalk);
+  // element j of this lane: wave-uniform base of column (i, j) plus the
+  // lane's byte offset (scalar base + 32-bit vector offset addressing)
+  const int64_t cbase = uniform64(S.off + (int64_t)i * S.ny * nz);
+  const unsigned kcb = (unsigned)kc * (unsigned)sizeof(double);
+  auto at = [&](const double* f, int j) { return *lane_ptr(f + cbase + (int64_t)j * nz, kcb); };
+  auto cls_at = [&](int j) { return __builtin_amdgcn_readfirstlane((int)v.cls[S.o_cls + (int64_t)i * S.ny + j]); };
+  // a value with its lane neighbours (levels k+1, k-1): each shifted once and
+  // shared by the z operator and the face products (the kernel is bound by
+  // instruction issue, and a 64-bit lane shift is two DPP moves)
+  struct Nb {
+    double c, p, m;
+  };
+  auto nb = [&](double q) { return Nb{q, wave_shl1(q), wave_shr1(q)}; };
+  auto face = [&](const Nb& a, const Nb& b) { return (a.p - a.c) * (b.p - b.c); };
+  double gy = 0.0, gz = 0.0;
+  if (la < lb) {
+    // element la - 1 (left neighbour of the first element and of the first face)
+    const int j0 = la > 0 ? la - 1 : la;
+    double m_prev = at(mu, j0);
+    double z_prev = 0.0, r_prev = 0.0;
+    if (EY && la > 0) {
+      const int cl = cls_at(j0);
+      const double ez = shZ[(cl * 64 + lane) * 2], fz = shZ[(cl * 64 + lane) * 2 + 1];
+      auto Az = [&](const Nb& q) { return ez * (q.p - q.c) - fz * (q.c - q.m); };
+      const double up = at(u, j0);
+      const Nb nu = nb(up);
+      const Nb n1 = nb(up + a1 * Az(nu));
+      const Nb n2 = nb(up + a2 * Az(n1));
+      z_prev = up + a3 * Az(n2);
+      r_prev = shR[cl * 64 + lane];
+    }
+    // operands in flight in 4-slot register rings (slot of element j: (j - la)
+    // % 4, the loop unrolled 4 times so every slot index is a constant and no
+    // value moves between registers while its load is in flight): mu three
+    // elements ahead (the explicit step needs mu_{j+1}), u and the
+    // coefficients two ahead
+    auto jc = [&](int j) { return j < N ? j : N - 1; };
+    double rm[4], ru[4], rea[4], reb[4], rhy[4], riy[4];
+    int rcl[4];
+    auto load_mu = [&](int slot, int j) { rm[slot & 3] = at(mu, jc(j)); };
+    auto load_rest = [&](int slot, int j) {
+      const int jj = jc(j);
+      ru[slot & 3] = at(u, jj);
+      rea[slot & 3] = EY ? *lane_ptr(v.coef + jj, ofa) : 0.0;
+      reb[slot & 3] = EY ? *lane_ptr(v.coef + jj, ofb) : 0.0;
+      rcl[slot & 3] = cls_at(jj);
+      rhy[slot & 3] = ag.hy[jj];
+      riy[slot & 3] = ag.iy[jc(j > 0 ? j - 1 : 0)];  // used for j > 0 only
+    };
+#pragma unroll
+    for (int q = 0; q < 3; ++q) load_mu(q, la + q);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) load_rest(q, la + q);
+    auto step = [&](int s, int j, bool act) __attribute__((always_inline)) {
+      load_mu(s + 3, j + 3);
+      load_rest(s + 2, j + 2);
+      const double m_cur = rm[s & 3], m_nx = rm[(s + 1) & 3], u_cur = ru[s & 3];
+      const int cl = rcl[s & 3];
+      const double ez = shZ[(cl * 64 + lane) * 2], fz = shZ[(cl * 64 + lane) * 2 + 1];
+      const double rk = shR[cl * 64 + lane];
+      auto Az = [&](const Nb& q) { return ez * (q.p - q.c) - fz * (q.c - q.m); };
+      const Nb nu = nb(u_cur);
+      const Nb n1 = nb(u_cur + a1 * Az(nu));
+      const Nb n2 = nb(u_cur + a2 * Az(n1));
+      double m = m_cur;
+      if (EY) {
+        const double z = u_cur + a3 * Az(n2);
+        const double fy = riy[s & 3] * (m_cur - m_prev) * (z - z_prev) * (r_prev + rk);
+        gy += (act && j > 0) ? fy : 0.0;
+        m = m_cur + c * (rea[s & 3] * (m_nx - m_cur) - reb[s & 3] * (m_cur - m_prev));
+        z_prev = z;
+        r_prev = rk;
+      }
+      const Nb nm = nb(m);
+      const Nb nm2 = nb(a3 * Az(nm));
+      const Nb nm1 = nb(a2 * Az(nm2));
+      const double o = m + nm2.c + nm1.c + a1 * Az(nm1);
+      const double fs = a3 * face(nm, n2) + a2 * face(nm2, n1) + a1 * face(nm1, nu);
+      const double vup = -hxi * rhy[s & 3] * izk * fs;
+      const double fz2 = 0.5 * rk * (vup + wave_shr1(vup));
+      gz += act ? fz2 : 0.0;
+      // the sums are completed here: otherwise the compiler defers every
+      // step's terms past the conditional stores and holds them all live
+      asm volatile("" : "+v"(gy), "+v"(gz));
+      if (outl && act) *lane_ptr(out + cbase + (int64_t)j * nz, kcb) = o;
+      m_prev = m_cur;
+      // keeps the scheduler from interleaving steps (which would hold several
+      // elements' values at once and spill)
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    // straight-line code over the wave's kWalk elements: a loop back-edge
+    // would have to move the rings' registers and so wait for every load in
+    // flight; steps past the wave's range read clamped elements and neither
+    // store nor add to the gradient
+#pragma unroll
+    for (int q = 0; q < kWalk; ++q) step(q & 3, la + q, la + q < lb);
+  }
+  const double acc = gz + (EY && kin ? -0.5 * c * hxi * ag.hz[kc] * gy : 0.0);
+  add_line_part(acc, ag.gy_part + (int64_t)p * S.nx * nz, T, nz, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -153,10 +211,10 @@ __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPi
   const int nl = R.part == 0 ? A.nt : A.ni;
   for (int t = threadIdx.x; t < nl; t += blockDim.x) {
     const int lev = k0 + t;
-    ap.partial[R.poff + t] = ordered_sum_adj<16>(R.ncol, [&](int c) {
-      const PipeCol pc = v.pcol[R.col0 + c];
-      const double m = mu[S.off + ((int64_t)pc.i * S.ny + pc.j) * S.nz + lev];
-      return cell_w(ag, v, S, pc.i, pc.j, lev) * m;
+    const double hz = ag.hz[lev];
+    ap.partial[R.poff + t] = ordered_sum_adj<kRedChunk>(R.ncol, [&](int c) {
+      const AdjCol ac = ap.col[R.col0 + c];
+      return ac.hxy * hz * ag.rho[(int64_t)ac.cls * S.nz + lev] * mu[ac.off + lev];
     });
   }
 }
@@ -164,44 +222,40 @@ __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPi
 // Pass 2: per borehole.  Lam_k = sum over the inner chunks (turnaround levels);
 // s_j = (1/n_an) sum over the turnaround entries of level j of w * Lam_k (the
 // adjoint of the annulus level means); the inlet gradient adds the annulus
-// inlet levels' sums.
-__global__ __launch_bounds__(256) void k_adj_final(DevView v, AdjPipe ap) {
+// inlet levels' sums.  Level sums in a fixed order (level_sum).
+__global__ __launch_bounds__(kFinThreads) void k_adj_final(DevView v, AdjPipe ap) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* lam = reinterpret_cast<double*>(smem_raw);  // [nt]
-  __shared__ double red[256];
+  __shared__ double red[kFinThreads];
   const int b = blockIdx.x;
   const BhDev B = v.bh[b];
   const AdjBh A = ap.bh[b];
-  const int t = threadIdx.x;
-  for (int q = t; q < A.nt; q += 256) {
-    const int lev = A.t0 + q;
-    const bool turn = v.dep[B.dep_off + 2 * lev].type == DEP_TURN;
-    lam[q] = turn ? ordered_sum_adj<16>(A.nred_in, [&](int c) {
-      return ap.partial[ap.chunk[A.red0 + c].poff + q];
-    }) : 0.0;
+  const int t = threadIdx.x, l = t % kFinLP, s = t / kFinLP;
+  // a borehole's chunk partials are contiguous: chunk c of a part at base + c * levels
+  const double* pin = ap.partial + (A.nred_in > 0 ? ap.chunk[A.red0].poff : 0);
+  const double* pan = ap.partial + (A.nred_an > 0 ? ap.chunk[A.red0 + A.nred_in].poff : 0);
+  for (int l0 = 0; l0 < A.nt; l0 += kFinLP) {
+    const int q = l0 + l;
+    const bool turn = q < A.nt && v.dep[B.dep_off + 2 * (A.t0 + q)].type == DEP_TURN;
+    const double x = level_sum(turn ? A.nred_in : 0, [&](int c) { return pin[(int64_t)c * A.nt + q]; }, red);
+    if (s == 0 && q < A.nt) lam[q] = x;
   }
   __syncthreads();
-  for (int q = t; q < B.nlev; q += 256) {
-    double s = 0.0;
+  for (int q = t; q < B.nlev; q += kFinThreads) {
+    double sm = 0.0;
     for (int e = ap.toff[A.turn_off + q]; e < ap.toff[A.turn_off + q + 1]; ++e)
-      s += ap.tent[e].w * lam[ap.tent[e].k - A.t0];
-    ap.sbuf[A.soff + q] = s / (double)B.n_an;
+      sm += ap.tent[e].w * lam[ap.tent[e].k - A.t0];
+    ap.sbuf[A.soff + q] = sm / (double)B.n_an;
   }
   double g = 0.0;
-  for (int q = t; q < A.ni; q += 256) {
-    const int lev = A.i0 + q;
-    if (v.dep[B.dep_off + 2 * lev + 1].type == DEP_INLET)
-      g += ordered_sum_adj<16>(A.nred_an, [&](int c) {
-        return ap.partial[ap.chunk[A.red0 + A.nred_in + c].poff + q];
-      });
+  for (int l0 = 0; l0 < A.ni; l0 += kFinLP) {
+    const int q = l0 + l;
+    const bool inl = q < A.ni && v.dep[B.dep_off + 2 * (A.i0 + q) + 1].type == DEP_INLET;
+    const double x = level_sum(inl ? A.nred_an : 0, [&](int c) { return pan[(int64_t)c * A.ni + q]; }, red);
+    if (s == 0) g += x;
   }
-  red[t] = g;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (t < h) red[t] += red[t + h];
-    __syncthreads();
-  }
-  if (t == 0) ap.gtin[b] += red[0];
+  g = block_sum<kFinThreads>(g, red);
+  if (t == 0) ap.gtin[b] += g;
 }
 
 // Pass 3: per advection group (up to 8 columns of one borehole and kind), for
@@ -221,14 +275,28 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
   const int t = threadIdx.x;
   const int* off = ap.off + (Gp.part == 0 ? A.csr_in : A.csr_an);
   const double sd = (Gp.part == 0 && seed) ? seed[Gp.bh] : 0.0;
+  // the gather lists do not depend on the column: the first two entries of
+  // each of the thread's levels in registers, loaded once per block (a level
+  // of a semi-Lagrangian shift gathers from at most a few departures)
+  int e_beg[R], e_end[R];
+  AdjEnt g0[R], g1[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k = t + 256 * r;
+    e_beg[r] = k < nlev ? off[k] : 0;
+    e_end[r] = k < nlev ? off[k + 1] : 0;
+    g0[r] = e_end[r] > e_beg[r] ? ap.ent[e_beg[r]] : AdjEnt{0, 0, 0.0};
+    g1[r] = e_end[r] > e_beg[r] + 1 ? ap.ent[e_beg[r] + 1] : AdjEnt{0, 0, 0.0};
+  }
   for (int c = 0; c < Gp.ncol; ++c) {
-    const PipeCol pc = v.pcol[Gp.col0 + c];
-    double* p = mu + S.off + ((int64_t)pc.i * S.ny + pc.j) * S.nz;
+    const AdjCol ac = ap.col[Gp.col0 + c];
+    double* p = mu + ac.off;
+    const double* rho = ag.rho + (int64_t)ac.cls * S.nz;
     double wk[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = t + 256 * r;
-      wk[r] = k < nlev ? cell_w(ag, v, S, pc.i, pc.j, k) : 1.0;
+      wk[r] = k < nlev ? ac.hxy * ag.hz[k] * rho[k] : 1.0;
       if (k < nlev) lout[k] = wk[r] * p[k];
     }
     __syncthreads();
@@ -236,8 +304,8 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
     for (int r = 0; r < R; ++r) {
       const int k = t + 256 * r;
       if (k < nlev) {
-        double lin = 0.0;
-        for (int e = off[k]; e < off[k + 1]; ++e) lin += ap.ent[e].w * lout[ap.ent[e].k];
+        double lin = g0[r].w * lout[g0[r].k] + g1[r].w * lout[g1[r].k];
+        for (int e = e_beg[r] + 2; e < e_end[r]; ++e) lin += ap.ent[e].w * lout[ap.ent[e].k];
         if (Gp.part == 1 && k >= B.klo && k < B.klo + B.nlev) lin += ap.sbuf[A.soff + k - B.klo];
         if (Gp.part == 0 && k < B.n_outlev) lin += sd;
         p[k] = lin / wk[r];
@@ -246,7 +314,7 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
     // outlet levels below the borehole bottom (v_inner dt/2 > h): cells the
     // advection leaves alone but the outlet mean still reads
     if (sd != 0.0)
-      for (int k = nlev + t; k < B.n_outlev; k += 256) p[k] += sd / cell_w(ag, v, S, pc.i, pc.j, k);
+      for (int k = nlev + t; k < B.n_outlev; k += 256) p[k] += sd / (ac.hxy * ag.hz[k] * rho[k]);
     __syncthreads();
   }
 }
@@ -262,22 +330,20 @@ __global__ __launch_bounds__(256) void k_mul_w(DevView v, AdjGeo ag, double* lam
   }
 }
 
-hipError_t launch_grad_stream(const DevView& v, const AdjGeo& ag, int dir, const double* mu1, const double* mu2,
-                              const double* x, double scale, hipStream_t st) {
-  const int64_t n = dir == 0 ? v.xcls[0][kLineClasses] : v.ycls[0][kLineClasses];
+hipError_t launch_adj_yz(const DevView& v, const AdjGeo& ag, bool ey, double* out, const double* mu,
+                         const double* u, double c, double dt, double c3, hipStream_t st) {
+  const int64_t n = v.hcls[0][kLineClasses];
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_grad_stream, dim3((unsigned)n), dim3(64 * kStreamWaves), 0, st, v, ag, dir, mu1, mu2, x,
-                     scale);
+  const double a1 = 2.0 * c3 * dt, a2 = 0.5 * dt, a3 = dt;
+  const dim3 grid((unsigned)(n * ag.yparts)), block(64 * kStreamWaves);
+  if (ey) hipLaunchKernelGGL(k_adj_yz<true>, grid, block, 0, st, v, ag, out, mu, u, c, a1, a2, a3);
+  else hipLaunchKernelGGL(k_adj_yz<false>, grid, block, 0, st, v, ag, out, mu, u, c, a1, a2, a3);
   return hipGetLastError();
 }
 
-hipError_t launch_adj_z(const DevView& v, const AdjGeo& ag, double* out, const double* mu, const double* u,
-                        double dt, double c3, hipStream_t st) {
-  const int64_t n = v.hcls[0][kLineClasses];
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_adj_z, dim3((unsigned)n), dim3(64 * kStreamWaves), 0, st, v, ag, out, mu, u, 2.0 * c3 * dt,
-                     0.5 * dt, dt);
-  return hipGetLastError();
+int adj_yparts(int64_t htiles, int ny) {
+  (void)htiles;
+  return std::max(1, (ny + kStreamWaves * kWalk - 1) / (kStreamWaves * kWalk));
 }
 
 hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap, double* mu, const double* seed,
@@ -288,8 +354,8 @@ hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_adj_final, dim3((unsigned)v.nbh), dim3(256), (size_t)v.countz_max * sizeof(double), st, v,
-                     ap);
+  hipLaunchKernelGGL(k_adj_final, dim3((unsigned)v.nbh), dim3(kFinThreads), (size_t)v.countz_max * sizeof(double), st,
+                     v, ap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (v.nagroup == 0) return hipSuccess;
@@ -298,7 +364,9 @@ hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap
   const dim3 grid((unsigned)v.nagroup), block(256);
   if (R <= 1) hipLaunchKernelGGL((k_adj_apply<1>), grid, block, sh, st, v, ag, ap, mu, seed);
   else if (R <= 2) hipLaunchKernelGGL((k_adj_apply<2>), grid, block, sh, st, v, ag, ap, mu, seed);
+  else if (R <= 3) hipLaunchKernelGGL((k_adj_apply<3>), grid, block, sh, st, v, ag, ap, mu, seed);
   else if (R <= 4) hipLaunchKernelGGL((k_adj_apply<4>), grid, block, sh, st, v, ag, ap, mu, seed);
+  else if (R <= 6) hipLaunchKernelGGL((k_adj_apply<6>), grid, block, sh, st, v, ag, ap, mu, seed);
   else if (R <= 8) hipLaunchKernelGGL((k_adj_apply<8>), grid, block, sh, st, v, ag, ap, mu, seed);
   else if (R <= 16) hipLaunchKernelGGL((k_adj_apply<16>), grid, block, sh, st, v, ag, ap, mu, seed);
   else return hipErrorInvalidValue;

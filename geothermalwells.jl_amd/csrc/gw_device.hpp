@@ -46,6 +46,53 @@ GW_DEV double wave_shl1(double v) {
 GW_DEV float wave_shr1(float v) { return __builtin_bit_cast(float, dpp_shr1(__builtin_bit_cast(uint32_t, v))); }
 GW_DEV float wave_shl1(float v) { return __builtin_bit_cast(float, dpp_shl1(__builtin_bit_cast(uint32_t, v))); }
 
+// Sum of x over the block with a fixed-order tree (deterministic for a given
+// block size NT); red holds NT doubles.
+template <int NT>
+GW_DEV double block_sum(double x, double* red) {
+  red[threadIdx.x] = x;
+  __syncthreads();
+#pragma unroll
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Level-sum layout of the block that combines the chunk partials: kFinLP
+// levels per pass, kFinS chunk slices.
+constexpr int kFinLP = 128, kFinS = 8, kFinThreads = kFinLP * kFinS;
+
+// Per-level sum of the chunk partials ld(0) .. ld(n-1) by a kFinThreads
+// block: thread (l, s) = (t % kFinLP, t / kFinLP) adds chunks s, s + kFinS, ...
+// in order, 4 loads in flight; the slices are then added in slice order.  The
+// result is valid in the threads with s == 0; every thread must call it.
+template <class F>
+GW_DEV double level_sum(int n, F ld, double* red) {
+  const int t = threadIdx.x, l = t % kFinLP, s = t / kFinLP;
+  double acc = 0.0;
+  int q = s;
+  for (; q + 3 * kFinS < n; q += 4 * kFinS) {
+    const double x0 = ld(q), x1 = ld(q + kFinS), x2 = ld(q + 2 * kFinS), x3 = ld(q + 3 * kFinS);
+    acc += x0;
+    acc += x1;
+    acc += x2;
+    acc += x3;
+  }
+  for (; q < n; q += kFinS) acc += ld(q);
+  red[t] = acc;
+  __syncthreads();
+  double r = 0.0;
+  if (s == 0)
+#pragma unroll
+    for (int ss = 0; ss < kFinS; ++ss) r += red[ss * kFinLP + l];
+  __syncthreads();
+  return r;
+}
+
 // Scenario s with prefix[s] <= idx < prefix[s+1].
 GW_DEV int find_scen(const int64_t* prefix, int n, int64_t idx) {
   int lo = 0, hi = n - 1;

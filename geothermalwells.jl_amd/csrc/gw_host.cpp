@@ -248,6 +248,7 @@ struct gw_ctx {
   int lw_sel[5][2] = {};
   int force_lw = 0;
   bool tuned = false;
+  int adj_lw[2] = {0, 0};  // k-window width of the adjoint line modes ADJ_Y, ADJ_X (0: not yet timed)
   int lw(int mode, int dir) const { return lw_sel[mode][dir]; }
   bool fused_ok = false;  // fused z-step line kernels usable (few enough column classes)
   std::map<double, CTab> ctabs;
@@ -563,6 +564,11 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   std::vector<Dep> dep;
   std::vector<BhDev> hb(ctx.nbh_total);
   std::vector<RedChunk> chunks, groups;
+  // columns per advection block: about 2048 blocks per launch, 1 to 8 columns
+  // each (a block walks its columns one after another)
+  int64_t npipe = 0;
+  for (int b = 0; b < ctx.nbh_total; ++b) npipe += ctx.bh_n_in[b] + ctx.bh_n_an[b];
+  const int kAdvGroup = (int)std::min<int64_t>(8, std::max<int64_t>(1, npipe / 2048));
   int gb = 0, mean_off = 0, poff = 0;
   for (size_t si = 0; si < ctx.sc.size(); ++si) {
     const Scen& s = ctx.sc[si];
@@ -633,8 +639,8 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       }
       B.mean_off = mean_off;
       mean_off += B.nlev;
-      // reduction chunks: 64 pipe columns each
-      constexpr int kChunk = 64;
+      // reduction chunks of kRedChunk pipe columns
+      constexpr int kChunk = kRedChunk;
       B.red_in0 = (int)chunks.size();
       for (int c0 = 0; c0 < B.n_in; c0 += kChunk) {
         const int nc = std::min(kChunk, B.n_in - c0);
@@ -651,7 +657,6 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
         }
       B.n_red_an = (int)chunks.size() - B.red_an0;
       // advection groups: kAdvGroup columns of one kind each
-      constexpr int kAdvGroup = 8;
       for (int c0 = 0; c0 < B.n_in; c0 += kAdvGroup)
         groups.push_back(RedChunk{gb, 0, B.col_in0 + c0, std::min(kAdvGroup, B.n_in - c0), 0, 0});
       for (int c0 = 0; c0 < B.n_an; c0 += kAdvGroup)
@@ -1355,6 +1360,25 @@ void grid_h(const std::vector<double>& g, std::vector<double>& h, std::vector<do
   }
 }
 
+// Geometry, materials and zeroed gradient buffers of the adjoint kernels.
+AdjGeo make_adj_geo(const gw_ctx& ctx, DevBufs& db) {
+  const Scen& s = ctx.sc[0];
+  std::vector<double> hx, hy, hz, ix, iy, iz;
+  grid_h(s.gx, hx, ix);
+  grid_h(s.gy, hy, iy);
+  grid_h(s.gz, hz, iz);
+  std::vector<double> rock(s.rockprof.begin(), s.rockprof.end());
+  AdjGeo ag{};
+  ag.hx = db.up(hx); ag.hy = db.up(hy); ag.hz = db.up(hz);
+  ag.ix = db.up(ix); ag.iy = db.up(iy); ag.iz = db.up(iz);
+  ag.rho = db.up(s.rprof);
+  ag.rock = db.up(rock);
+  ag.yparts = adj_yparts(ctx.hcls[0][kLineClasses], s.ny);
+  ag.gx_part = db.up(std::vector<double>((size_t)s.ny * s.nz, 0.0));
+  ag.gy_part = db.up(std::vector<double>((size_t)ag.yparts * s.nx * s.nz, 0.0));
+  return ag;
+}
+
 // Transposed advection tables of one step length (departure table of c).
 AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
   const Scen& s = ctx.sc[0];
@@ -1364,7 +1388,7 @@ AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
   std::vector<AdjEnt> ent, tent;
   std::vector<RedChunk> ch;
   int poff = 0, soff = 0;
-  constexpr int kChunk = 64;
+  constexpr int kChunk = kRedChunk;
   for (int b = 0; b < ctx.nbh_total; ++b) {
     const BhDev& B = tab.h_bh[b];
     AdjBh& A = abh[b];
@@ -1430,8 +1454,21 @@ AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
       }
     A.nred_an = (int32_t)ch.size() - A.red0 - A.nred_in;
   }
+  // per pipe column: first cell, class, hx * hy
+  std::vector<double> hx, hy, ix, iy;
+  grid_h(s.gx, hx, ix);
+  grid_h(s.gy, hy, iy);
+  std::vector<AdjCol> cols(ctx.pcol.size());
+  for (size_t q = 0; q < ctx.pcol.size(); ++q) {
+    const PipeCol& pc = ctx.pcol[q];
+    cols[q].off = ((int64_t)pc.i * s.ny + pc.j) * s.nz;
+    cols[q].hxy = hx[pc.i] * hy[pc.j];
+    cols[q].cls = s.cls[(size_t)pc.j + (size_t)s.ny * pc.i];
+    cols[q].pad = 0;
+  }
   AdjPipe ap{};
   ap.bh = db.up(abh);
+  ap.col = db.up(cols);
   ap.off = db.up(off);
   ap.ent = db.up(ent);
   ap.toff = db.up(toff);
@@ -1457,25 +1494,62 @@ int run_fused(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, int 
   return fused_rest(ctx, tab, (n & 1) ? B : phi, phi, t + n * dt, dt, true, jacc);
 }
 
-// One step with the separate operators from u, keeping z = R(dt A_z) u and
-// e = (I - cA_x)^-1 b; u_next receives the step's result.
-int adj_recompute(gw_ctx& ctx, const CTab* tab, const double* u, double* z, double* e, double* u_next,
-                  double* wb, double t_end, double dt) {
+// One step from u with the fused z-step + explicit y kernel, the fused x
+// kernel keeping e = (I - cA_x)^-1 b, and the implicit y; u_next receives the
+// step's result.
+int adj_recompute(gw_ctx& ctx, const CTab* tab, const double* u, double* e, double* u_next, double* wb,
+                  double t_end, double dt) {
   const double c = dt / 2;
   const DevView v = ctx.view(tab);
-  const size_t bytes = (size_t)ctx.cells_total * sizeof(double);
-  int rc = hip_status(hipMemcpyAsync(z, u, bytes, hipMemcpyDeviceToDevice, ctx.stream), "copy");
-  if (!rc) rc = hip_status(launch_zstep(v, GW_F64, z, dt, ctx.c3, ctx.stream), "zstep");
-  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), wb, z, c, 0.0, 0.0,
-                                       ctx.stream), "explicit y");
+  int rc = hip_status(launch_line(v, GW_F64, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), wb, u, c, dt, ctx.c3, ctx.stream),
+                      "z-step + explicit y");
   if (!rc) rc = sub_step_advect(ctx, tab, wb, t_end);
-  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 0, ctx.lw(LINE_IMPLICIT, 0), e, wb, c, 0.0, 0.0,
-                                       ctx.stream), "implicit x");
-  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 0, ctx.lw(LINE_EXPLICIT, 0), wb, e, c, 0.0, 0.0,
-                                       ctx.stream), "explicit x");
+  DevView vk = v;
+  vk.aux = e;
+  if (!rc) rc = hip_status(launch_line(vk, GW_F64, LINE_IMPLICIT_EXPLICIT_KEEP, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0),
+                                       wb, wb, c, 0.0, 0.0, ctx.stream), "implicit + explicit x");
   if (!rc) rc = sub_step_advect(ctx, tab, wb, t_end + c);
   if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), u_next, wb, c, 0.0,
                                        0.0, ctx.stream), "implicit y");
+  return rc;
+}
+
+// The adjoint line modes' k-window widths: forced by the problem flags, else
+// each mode timed once per context in both widths (out of place on scratch
+// fields; the gradient buffers it touches are cleared by the caller).
+int adj_tune(gw_ctx& ctx, const CTab* tab, const AdjGeo& ag, double* out, const double* in, const double* x) {
+  if (ctx.adj_lw[0]) return GW_OK;
+  if (ctx.force_lw) {
+    ctx.adj_lw[0] = ctx.adj_lw[1] = ctx.force_lw;
+    return GW_OK;
+  }
+  const DevView v = ctx.view(tab);
+  const double c = tab->c;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = hip_status(hipEventCreate(&e0), "event");
+  if (!rc) rc = hip_status(hipEventCreate(&e1), "event");
+  for (int mode = 0; mode < 2 && !rc; ++mode) {
+    float best = 0.f;
+    int best_lw = 64;
+    for (int lw : {64, 32}) {
+      auto run = [&]() {
+        return hip_status(lw == 32 ? line_adj_w32(v, ag, mode, out, in, x, nullptr, c, 0.0, 0.0, 0.0, ctx.stream)
+                                   : line_adj_w64(v, ag, mode, out, in, x, nullptr, c, 0.0, 0.0, 0.0, ctx.stream),
+                          "adjoint line tune");
+      };
+      if ((rc = run())) break;
+      if ((rc = hip_status(hipEventRecord(e0, ctx.stream), "event"))) break;
+      for (int r = 0; r < 5 && !rc; ++r) rc = run();
+      if (rc || (rc = hip_status(hipEventRecord(e1, ctx.stream), "event"))) break;
+      if ((rc = hip_status(hipEventSynchronize(e1), "event sync"))) break;
+      float ms = 0.f;
+      if ((rc = hip_status(hipEventElapsedTime(&ms, e0, e1), "event time"))) break;
+      if (best == 0.f || ms < best) { best = ms; best_lw = lw; }
+    }
+    ctx.adj_lw[mode] = best_lw;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   return rc;
 }
 
@@ -1484,31 +1558,21 @@ int adj_recompute(gw_ctx& ctx, const CTab* tab, const double* u, double* z, doub
 extern "C" {
 
 int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void* x, double dt) {
-  if (!ctx || !out || !in || op < 0 || op > 5) return fail(GW_EINVAL, "bad argument");
+  if (!ctx || !out || !in || op < 0 || op > 6) return fail(GW_EINVAL, "bad argument");
   if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
   if (ctx->sc.size() != 1 || ctx->prec != GW_F64 || !ctx->fused_ok)
     return fail(GW_EINVAL, "the adjoint operators take one FP64 scenario");
-  if ((op == 1 || op == 2 || op == 4) && !x) return fail(GW_EINVAL, "this operator needs the forward field x");
+  if ((op == 1 || op == 2 || op == 4 || op == 6) && !x)
+    return fail(GW_EINVAL, "this operator needs the forward field x");
+  if ((op == 4 || op == 6) && (out == in || out == x)) return fail(GW_EINVAL, "out must not alias in or x");
   gw_ctx& C = *ctx;
-  const Scen& s = C.sc[0];
   try {
     CTab* tab = nullptr;
     int rc = prepare_steps(C, in, dt, tab);
     if (rc) return rc;
     const double c = dt / 2;
     DevBufs db;
-    std::vector<double> hx, hy, hz, ix, iy, iz;
-    grid_h(s.gx, hx, ix);
-    grid_h(s.gy, hy, iy);
-    grid_h(s.gz, hz, iz);
-    std::vector<double> rock(s.rockprof.begin(), s.rockprof.end());
-    AdjGeo ag{};
-    ag.hx = db.up(hx); ag.hy = db.up(hy); ag.hz = db.up(hz);
-    ag.ix = db.up(ix); ag.iy = db.up(iy); ag.iz = db.up(iz);
-    ag.rho = db.up(s.rprof);
-    ag.rock = db.up(rock);
-    ag.gx_part = db.up(std::vector<double>((size_t)s.ny * s.nz, 0.0));
-    ag.gy_part = db.up(std::vector<double>((size_t)s.nx * s.nz, 0.0));
+    const AdjGeo ag = make_adj_geo(C, db);
     const DevView v = C.view(tab);
     double* o = static_cast<double*>(out);
     const double* i = static_cast<const double*>(in);
@@ -1524,7 +1588,8 @@ int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void
       case 1:
       case 2: {
         const int mode = op == 1 ? ADJ_Y : ADJ_X;
-        const int lw = mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1) : C.lw(LINE_IMPLICIT_EXPLICIT, 0);
+        const int lw = C.adj_lw[mode] ? C.adj_lw[mode] : (mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1)
+                                                                         : C.lw(LINE_IMPLICIT_EXPLICIT, 0));
         rc = hip_status(lw == 32 ? line_adj_w32(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream)
                                  : line_adj_w64(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream),
                         "adjoint line");
@@ -1535,7 +1600,8 @@ int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void
                                     C.stream), "explicit y");
         break;
       case 4:
-        rc = hip_status(launch_adj_z(v, ag, o, i, xf, dt, C.c3, C.stream), "adjoint z");
+      case 6:
+        rc = hip_status(launch_adj_yz(v, ag, op == 6, o, i, xf, c, dt, C.c3, C.stream), "adjoint z");
         break;
       default:
         rc = hip_status(launch_mul_w(v, ag, o, i, C.stream), "W mu");
@@ -1571,27 +1637,14 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     const size_t cells = (size_t)C.cells_total, bytes = cells * sizeof(double);
     const int nbh = C.nbh_total;
     DevBufs db;
-    // geometry and material tables
-    std::vector<double> hx, hy, hz, ix, iy, iz;
-    grid_h(s.gx, hx, ix);
-    grid_h(s.gy, hy, iy);
-    grid_h(s.gz, hz, iz);
-    std::vector<double> rock(s.rockprof.begin(), s.rockprof.end());
-    AdjGeo ag{};
-    ag.hx = db.up(hx); ag.hy = db.up(hy); ag.hz = db.up(hz);
-    ag.ix = db.up(ix); ag.iy = db.up(iy); ag.iz = db.up(iz);
-    ag.rho = db.up(s.rprof);
-    ag.rock = db.up(rock);
-    ag.gx_part = db.up(std::vector<double>((size_t)s.ny * s.nz, 0.0));
-    ag.gy_part = db.up(std::vector<double>((size_t)s.nx * s.nz, 0.0));
+    const AdjGeo ag = make_adj_geo(C, db);
     const AdjPipe ap = build_adj_pipe(C, *tab, db);
-    // storage: checkpoints, one segment of u / z / e, two adjoint fields, a work field
+    // storage: checkpoints, one segment of u / e, two adjoint fields, a work field
     const int nseg = (nsteps + seg - 1) / seg;
     const int sl = std::min(seg, nsteps);
-    std::vector<double*> ck(nseg), U(sl + 1), Z(sl), E(sl);
+    std::vector<double*> ck(nseg), U(sl + 1), E(sl);
     for (auto& p : ck) p = db.alloc<double>(cells);
     for (auto& p : U) p = db.alloc<double>(cells);
-    for (auto& p : Z) p = db.alloc<double>(cells);
     for (auto& p : E) p = db.alloc<double>(cells);
     double* MU0 = db.alloc<double>(cells);
     double* MU1 = db.alloc<double>(cells);
@@ -1622,37 +1675,40 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     // backward
     const DevView v = C.view(tab);
     if ((rc = hip_status(hipMemsetAsync(MU0, 0, bytes, C.stream), "memset"))) return rc;
+    if ((rc = adj_tune(C, tab, ag, MU1, MU0, static_cast<const double*>(phi0)))) return rc;
+    if ((rc = hip_status(hipMemsetAsync(ag.gx_part, 0, sizeof(double) * (size_t)s.ny * s.nz, C.stream), "memset")) ||
+        (rc = hip_status(hipMemsetAsync(ag.gy_part, 0, sizeof(double) * (size_t)ag.yparts * s.nx * s.nz, C.stream),
+                         "memset")))
+      return rc;
     auto adj_line = [&](int mode, double* out, const double* in, const double* x1) {
-      const int lw = mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1) : C.lw(LINE_IMPLICIT_EXPLICIT, 0);
+      const int lw = C.adj_lw[mode];
       return hip_status(lw == 32 ? line_adj_w32(v, ag, mode, out, in, x1, nullptr, c, 0.0, 0.0, 0.0, C.stream)
                                  : line_adj_w64(v, ag, mode, out, in, x1, nullptr, c, 0.0, 0.0, 0.0, C.stream),
                         "adjoint line");
     };
+    double* A = MU0;  // mu_{m+1}, then mu_m
+    double* Bf = MU1;
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n = std::min(seg, nsteps - sg * seg);
       const int base = sg * seg;
       if ((rc = hip_status(hipMemcpyAsync(U[0], ck[sg], bytes, hipMemcpyDeviceToDevice, C.stream), "copy")))
         return rc;
       for (int m = 0; m < n; ++m)
-        if ((rc = adj_recompute(C, tab, U[m], Z[m], E[m], U[m + 1], WB, t0 + (double)(base + m + 1) * dt, dt)))
-          return rc;
+        if ((rc = adj_recompute(C, tab, U[m], E[m], U[m + 1], WB, t0 + (double)(base + m + 1) * dt, dt))) return rc;
       for (int m = n - 1; m >= 0; --m) {
-        // mu_{n+1} in MU0 -> mu_n in MU0
-        if ((rc = adj_line(ADJ_Y, MU1, MU0, U[m + 1]))) return rc;
-        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, MU1, seed, C.stream), "adjoint advection"))) return rc;
-        if ((rc = adj_line(ADJ_X, MU0, MU1, E[m]))) return rc;
-        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, MU0, nullptr, C.stream), "adjoint advection"))) return rc;
-        if ((rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 1, C.lw(LINE_EXPLICIT, 1), MU1, MU0, c, 0.0, 0.0,
-                                         C.stream), "explicit y")))
+        if ((rc = adj_line(ADJ_Y, Bf, A, U[m + 1]))) return rc;
+        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, Bf, seed, C.stream), "adjoint advection"))) return rc;
+        if ((rc = adj_line(ADJ_X, A, Bf, E[m]))) return rc;
+        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, A, nullptr, C.stream), "adjoint advection"))) return rc;
+        if ((rc = hip_status(launch_adj_yz(v, ag, true, Bf, A, U[m], c, dt, C.c3, C.stream), "adjoint y + z")))
           return rc;
-        if ((rc = hip_status(launch_grad_stream(v, ag, 1, MU0, nullptr, Z[m], c, C.stream), "gradient y"))) return rc;
-        if ((rc = hip_status(launch_adj_z(v, ag, MU0, MU1, U[m], dt, C.c3, C.stream), "adjoint z"))) return rc;
+        std::swap(A, Bf);
       }
     }
-    if (dJ_du0 && (rc = hip_status(launch_mul_w(v, ag, static_cast<double*>(dJ_du0), MU0, C.stream), "W mu")))
+    if (dJ_du0 && (rc = hip_status(launch_mul_w(v, ag, static_cast<double*>(dJ_du0), A, C.stream), "W mu")))
       return rc;
     // gradients to the host: per level over the lines in order, then per layer
-    std::vector<double> gxp((size_t)s.ny * s.nz), gyp((size_t)s.nx * s.nz), gt(nbh);
+    std::vector<double> gxp((size_t)s.ny * s.nz), gyp((size_t)ag.yparts * s.nx * s.nz), gt(nbh);
     if ((rc = hip_status(hipMemcpyAsync(gxp.data(), ag.gx_part, sizeof(double) * gxp.size(), hipMemcpyDeviceToHost,
                                         C.stream), "copy")) ||
         (rc = hip_status(hipMemcpyAsync(gyp.data(), ag.gy_part, sizeof(double) * gyp.size(), hipMemcpyDeviceToHost,
@@ -1665,7 +1721,7 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     for (int k = 0; k < s.nz; ++k) {
       double tot = 0.0;
       for (int j = 0; j < s.ny; ++j) tot += gxp[(size_t)j * s.nz + k];
-      for (int i = 0; i < s.nx; ++i) tot += gyp[(size_t)i * s.nz + k];
+      for (int q = 0; q < ag.yparts * s.nx; ++q) tot += gyp[(size_t)q * s.nz + k];
       int lay = 0;
       if (!s.mat.depth.empty()) {
         lay = nlay - 1;

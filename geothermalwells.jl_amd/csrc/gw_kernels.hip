@@ -103,21 +103,6 @@ __global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, 
 // ---------------------------------------------------------------------------
 // Pipe kernels.
 // ---------------------------------------------------------------------------
-template <int NT>
-GW_DEV double block_sum(double x, double* red) {
-  // fixed-order tree: deterministic for a given block size
-  red[threadIdx.x] = x;
-  __syncthreads();
-#pragma unroll
-  for (int h = NT / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
-}
-
 // Sum of ld(0) .. ld(n-1) added in index order, NB loads in flight at a time
 // (the padding of the last batch adds +0.0, which leaves the sum unchanged).
 template <int NB, class F>
@@ -134,20 +119,27 @@ GW_DEV double ordered_sum(int n, F ld) {
 }
 
 // Outlet mean, turnaround level means and inlet value of borehole b from the
-// chunk partials: chunks are combined in chunk order and the outlet levels
-// with a fixed tree, so the result is deterministic.
+// chunk partials (level_sum: a fixed order, so the result is deterministic).
 GW_DEV void pipe_final(const DevView& v, int b, double* red) {
   const BhDev B = v.bh[b];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, l = t % kFinLP, s = t / kFinLP;
+  // a borehole's chunk partials are contiguous: chunk q of a part at base + q * levels
+  const double* pin = v.partial + (B.n_red_in > 0 ? v.rchunk[B.red_in0].poff : 0);
+  const double* pan = v.partial + (B.n_red_an > 0 ? v.rchunk[B.red_an0].poff : 0);
   double acc = 0.0;
-  for (int lev = t; lev < B.n_outlev; lev += 256)
-    acc += ordered_sum<16>(B.n_red_in, [&](int q) { return v.partial[v.rchunk[B.red_in0 + q].poff + lev]; });
-  const double tot = block_sum<256>(acc, red);
+  for (int l0 = 0; l0 < B.n_outlev; l0 += kFinLP) {
+    const int lev = l0 + l;
+    const double x = level_sum(lev < B.n_outlev ? B.n_red_in : 0,
+                               [&](int q) { return pin[(int64_t)q * B.n_outlev + lev]; }, red);
+    if (s == 0) acc += x;
+  }
+  const double tot = block_sum<kFinThreads>(acc, red);
   const double tout = tot / ((double)B.n_in * (double)B.n_outlev);
-  for (int lev = t; lev < B.nlev; lev += 256) {
-    const double s =
-        ordered_sum<16>(B.n_red_an, [&](int q) { return v.partial[v.rchunk[B.red_an0 + q].poff + lev]; });
-    v.means[B.mean_off + lev] = s / (double)B.n_an;
+  for (int l0 = 0; l0 < B.nlev; l0 += kFinLP) {
+    const int lev = l0 + l;
+    const double x = level_sum(lev < B.nlev ? B.n_red_an : 0,
+                               [&](int q) { return pan[(int64_t)q * B.nlev + lev]; }, red);
+    if (s == 0 && lev < B.nlev) v.means[B.mean_off + lev] = x / (double)B.n_an;
   }
   if (t == 0) {
     v.tout[b] = tout;
@@ -159,11 +151,11 @@ GW_DEV void pipe_final(const DevView& v, int b, double* red) {
   }
 }
 
-// Pass 1: one block per reduction chunk (up to 64 pipe columns of one
+// Pass 1: one block per reduction chunk (up to kRedChunk pipe columns of one
 // borehole); thread t owns level t (coalesced within each column) and sums the
-// chunk's columns in order, 32 columns' loads in flight at a time.  (A single
-// pass with a last-block combine would need device-scope fences, which on this
-// multi-XCD part write back and invalidate the whole L2 per block.)
+// chunk's columns in order, all loads in flight at once.  (A single pass with
+// a last-block combine would need device-scope fences, which on this multi-XCD
+// part write back and invalidate the whole L2 per block.)
 template <class TS>
 __global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __restrict__ phi) {
   const RedChunk R = v.rchunk[blockIdx.x];
@@ -173,7 +165,7 @@ __global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __res
   const int nl = R.part == 0 ? B.n_outlev : B.nlev;
   const TS* base = phi + S.off + k0;
   for (int lev = threadIdx.x; lev < nl; lev += blockDim.x)
-    v.partial[R.poff + lev] = ordered_sum<32>(R.ncol, [&](int c) {
+    v.partial[R.poff + lev] = ordered_sum<kRedChunk>(R.ncol, [&](int c) {
       const PipeCol pc = v.pcol[R.col0 + c];
       return (double)base[((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz + lev];
     });
@@ -181,8 +173,8 @@ __global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __res
 
 // Pass 2: one block per borehole; jacc (may be NULL) accumulates the outlet
 // of every launch per borehole (the block owns its slot).
-__global__ __launch_bounds__(256) void k_pipe_final(DevView v, double* jacc) {
-  __shared__ double red[256];
+__global__ __launch_bounds__(kFinThreads) void k_pipe_final(DevView v, double* jacc) {
+  __shared__ double red[kFinThreads];
   pipe_final(v, blockIdx.x, red);
   if (jacc && threadIdx.x == 0) jacc[blockIdx.x] += v.tout[blockIdx.x];
 }
@@ -318,7 +310,7 @@ hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipSt
     hipLaunchKernelGGL((k_pipe_partial<float>), dim3(v.nrchunk), dim3(256), 0, st, v, (const float*)phi);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_pipe_final, dim3(v.nbh), dim3(256), 0, st, v, jacc);
+  hipLaunchKernelGGL(k_pipe_final, dim3(v.nbh), dim3(kFinThreads), 0, st, v, jacc);
   return hipGetLastError();
 }
 
@@ -329,7 +321,9 @@ static hipError_t advect_t(const DevView& v, TS* phi, hipStream_t st) {
   const int R = (v.countz_max + 255) / 256;
   if (R <= 1) hipLaunchKernelGGL((k_pipe_advect<TS, 1>), grid, block, sh, st, v, phi);
   else if (R <= 2) hipLaunchKernelGGL((k_pipe_advect<TS, 2>), grid, block, sh, st, v, phi);
+  else if (R <= 3) hipLaunchKernelGGL((k_pipe_advect<TS, 3>), grid, block, sh, st, v, phi);
   else if (R <= 4) hipLaunchKernelGGL((k_pipe_advect<TS, 4>), grid, block, sh, st, v, phi);
+  else if (R <= 6) hipLaunchKernelGGL((k_pipe_advect<TS, 6>), grid, block, sh, st, v, phi);
   else if (R <= 8) hipLaunchKernelGGL((k_pipe_advect<TS, 8>), grid, block, sh, st, v, phi);
   else if (R <= 16) hipLaunchKernelGGL((k_pipe_advect<TS, 16>), grid, block, sh, st, v, phi);
   else return hipErrorInvalidValue;

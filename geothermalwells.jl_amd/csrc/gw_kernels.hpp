@@ -18,6 +18,10 @@ inline int line_class(int n) {
   return -1;
 }
 
+// Pipe columns per reduction chunk (one partial sum per level).  Fixed per
+// borehole, so the summation order does not depend on the batch.
+constexpr int kRedChunk = 16;
+
 struct DevView {
   const ScenDev* sc;
   int32_t nscen;
@@ -54,9 +58,20 @@ struct DevView {
   const double* inlet_tab;      // [nbh] per-well table
   int32_t nx_max, ny_max, nz_max, countz_max;
   int64_t cells_total, cols_total;
+  void* aux;                    // LINE_IMPLICIT_EXPLICIT_KEEP: receives the implicit result
 };
 
-enum LineMode { LINE_EXPLICIT = 0, LINE_IMPLICIT = 1, LINE_IMPLICIT_EXPLICIT = 2, LINE_ZY = 3, LINE_YZY = 4 };
+// LINE_IMPLICIT_EXPLICIT_KEEP is LINE_IMPLICIT_EXPLICIT that also writes the
+// implicit result to DevView::aux (FP64 only; the adjoint's recompute keeps
+// it).  It shares the k-window width chosen for LINE_IMPLICIT_EXPLICIT.
+enum LineMode {
+  LINE_EXPLICIT = 0,
+  LINE_IMPLICIT = 1,
+  LINE_IMPLICIT_EXPLICIT = 2,
+  LINE_ZY = 3,
+  LINE_YZY = 4,
+  LINE_IMPLICIT_EXPLICIT_KEEP = 5
+};
 
 // ---------------------------------------------------------------------------
 // Adjoint (one scenario, FP64).  The 1-D operators are A = D^-1 K with K
@@ -75,7 +90,9 @@ struct AdjGeo {
   const double* rho;     // [ncls][nz] volumetric heat capacity per column class
   const double* rock;    // [ncls][nz] 1 where the conductivity is the rock layer's, else 0
   double* gx_part;       // [ny][nz] conductivity-gradient sums of the x-line kernels (per cell level)
-  double* gy_part;       // [nx][nz] ... of the y-line kernels (y faces and z faces)
+  double* gy_part;       // [yparts][nx][nz] ... of the y-line kernels (y faces and z faces); the
+                         // line kernels use part 0, the streaming kernel one part per block of a line
+  int32_t yparts;        // blocks per y-line tile of the streaming kernel (adj_yparts)
 };
 
 // Adjoint line modes (mu in, mu out; gradients added to the part buffers):
@@ -92,16 +109,14 @@ hipError_t line_adj_w32(const DevView& v, const AdjGeo& ag, int mode, double* ou
                         const double* x1, const double* x2, double c, double a1, double a2, double a3,
                         hipStream_t st);
 
-// Streaming adjoint kernels (no line held in registers): per tile of levels
-// (lanes) one pass along the line.
-//   grad_stream: grad += scale * G_dir(mu1 + mu2, x)   (mu2 may be NULL)
-//   adj_z:       out = R(dt A_z) mu (the transposed z-step in mu), and the
-//                z-face gradient of its three Horner stages with the forward
-//                stage values recomputed from u
-hipError_t launch_grad_stream(const DevView& v, const AdjGeo& ag, int dir, const double* mu1, const double* mu2,
-                              const double* x, double scale, hipStream_t st);
-hipError_t launch_adj_z(const DevView& v, const AdjGeo& ag, double* out, const double* mu, const double* u,
-                        double dt, double c3, hipStream_t st);
+// Streaming adjoint kernel (no line held in registers) on the halo tiles of
+// the y-lines: out = R(dt A_z) E_y mu (ey) or R(dt A_z) mu, the transposes in
+// mu of the forward z-step (from u) followed by the explicit y half-step, with
+// the z-face gradient of the three Horner stages and (ey) the explicit-y
+// gradient c G_y(mu, R(dt A_z) u); out != mu.  adj_yparts: blocks per tile.
+hipError_t launch_adj_yz(const DevView& v, const AdjGeo& ag, bool ey, double* out, const double* mu,
+                         const double* u, double c, double dt, double c3, hipStream_t st);
+int adj_yparts(int64_t htiles, int ny);
 
 // Transposed advection of one sub-step, in mu: per borehole, gather lists of
 // the departure table (CSR over the destination level), the turnaround
@@ -119,8 +134,16 @@ struct AdjBh {
   int32_t i0, ni;           // annulus levels reduced for the inlet (i0 .. i0+ni-1)
   int32_t red0, nred_in, nred_an, soff;  // chunks; offset of the annulus-level sums in sbuf
 };
+// One pipe column for the adjoint kernels: its first cell, column class and
+// hx * hy (W at level k = hxy hz[k] rho[cls][k]).
+struct AdjCol {
+  int64_t off;
+  double hxy;
+  int32_t cls, pad;
+};
 struct AdjPipe {
   const AdjBh* bh;
+  const AdjCol* col;        // per pipe column (the pipe column list's order)
   const int32_t* off;       // CSR offsets
   const AdjEnt* ent;        // CSR entries (source level, weight)
   const int32_t* toff;      // turnaround CSR offsets (per annulus level)

@@ -55,8 +55,10 @@ constexpr int kZcMax = 16;     // column classes staged for the fused z-step
 
 template <int MODE> struct ModeTraits {
   static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY;
-  static constexpr bool impl = MODE == LINE_IMPLICIT || MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_YZY;
+  static constexpr bool impl = MODE == LINE_IMPLICIT || MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_YZY ||
+                               MODE == LINE_IMPLICIT_EXPLICIT_KEEP;
   static constexpr bool expl = MODE != LINE_IMPLICIT;
+  static constexpr bool keep = MODE == LINE_IMPLICIT_EXPLICIT_KEEP;
   // coefficient arrays staged per set slot: bd, fw, bu (implicit) then ea, eb
   // (explicit), each NMAX long
   static constexpr int NA = (impl ? 3 : 0) + (expl ? 2 : 0);
@@ -282,6 +284,13 @@ __global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS
     }
   }
 
+  if (MT::keep && k >= T.ko0 && k < T.ko1) {
+    TS* pk = static_cast<TS*>(v.aux) + useg;
+#pragma unroll
+    for (int r = 0; r < S; ++r)
+      if (l0 + r < N) *lane_ptr(pk + (int64_t)r * ustride, kb) = (TS)d[r];
+  }
+
   if (MT::halo) {
     // z-step R(dt A_z) across lanes (lane = k), column class per line element;
     // neighbours k-1 / k+1 come from lanes -1 / +1 by DPP wave shifts (the
@@ -396,6 +405,12 @@ static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const T
       break;
     case LINE_ZY: line_launch<Cfg, TS, LINE_ZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
     case LINE_YZY: line_launch<Cfg, TS, LINE_YZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
+    case LINE_IMPLICIT_EXPLICIT_KEEP:
+      if constexpr (std::is_same<TS, double>::value) {
+        line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT_KEEP>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+        break;
+      }
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

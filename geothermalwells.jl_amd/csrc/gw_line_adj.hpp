@@ -128,29 +128,42 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
   double acc = 0.0;
 
   // sum over this segment's faces along the line of (mu2-mu1)(x2-x1) times
-  // the half rock weights of the two cells, times -c * phi_line
-  auto grad_line = [&](const double* x) __attribute__((always_inline)) {
-    shA[seg * LW + kk] = d[0];
+  // the half rock weights of the two cells, times -c * phi_line (mu: dd)
+  auto grad_line = [&](const double(&dd)[S], const double* x) __attribute__((always_inline)) {
+    shA[seg * LW + kk] = dd[0];
     __syncthreads();
     const double mu_edge = (l0 + S < N) ? shA[(seg + 1) * LW + kk] : 0.0;
     __syncthreads();
     const double* px = x + useg;
-    double xr = (l0 < N) ? *lane_ptr(px, kb) : 0.0;
-    double xn = (l0 + 1 < N) ? *lane_ptr(px + ustride, kb) : 0.0;
+    // x in chunks of KG elements, the next chunk's loads in flight while the
+    // current one computes; the barrier keeps the compiler from hoisting every
+    // load of the segment (which would spill the line)
+    constexpr int KG = S % 8 == 0 ? 8 : 4;
+    const double xafter = (l0 + S < N) ? *lane_ptr(px + (int64_t)S * ustride, kb) : 0.0;
+    double xn[KG];
+#pragma unroll
+    for (int q = 0; q < KG; ++q) xn[q] = (l0 + q < N) ? *lane_ptr(px + (int64_t)q * ustride, kb) : 0.0;
     double gsum = 0.0;
 #pragma unroll
-    for (int r = 0; r < S; ++r) {
-      const int l = l0 + r;
-      // the next element's load is issued one element ahead; the barrier keeps
-      // the compiler from hoisting all of them (which would spill the line)
-      const double xnn = (l + 2 < N && r + 2 <= S) ? *lane_ptr(px + (int64_t)(r + 2) * ustride, kb) : 0.0;
-      const double mn = (r + 1 < S) ? d[r + 1] : mu_edge;
-      if (l + 1 < N) {
-        const double wgt = shR[shK[l] * LW + kk] + shR[shK[l + 1] * LW + kk];
-        gsum += shI[l] * (mn - d[r]) * (xn - xr) * wgt;
+    for (int r0 = 0; r0 < S; r0 += KG) {
+      double xc[KG];
+#pragma unroll
+      for (int q = 0; q < KG; ++q) xc[q] = xn[q];
+      if (r0 + KG < S) {
+#pragma unroll
+        for (int q = 0; q < KG; ++q)
+          xn[q] = (l0 + r0 + KG + q < N) ? *lane_ptr(px + (int64_t)(r0 + KG + q) * ustride, kb) : 0.0;
       }
-      xr = xn;
-      xn = xnn;
+#pragma unroll
+      for (int q = 0; q < KG; ++q) {
+        const int r = r0 + q, l = l0 + r;
+        const double xnb = q + 1 < KG ? xc[q + 1] : (r0 + KG < S ? xn[0] : xafter);
+        const double mn = (r + 1 < S) ? dd[r + 1] : mu_edge;
+        if (l + 1 < N) {
+          const double wgt = shR[shK[l] * LW + kk] + shR[shK[l + 1] * LW + kk];
+          gsum += shI[l] * (mn - dd[r]) * (xnb - xc[q]) * wgt;
+        }
+      }
       asm volatile("" ::: "memory");
     }
     acc += -0.5 * c * phi_line * gsum;
@@ -287,12 +300,23 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
 
   if (MODE == ADJ_Y) {
     thomas();
-    grad_line(x1);
-  } else {
-    grad_line(x1);
+    grad_line(d, x1);
+  } else if (S <= 32) {
+    // G(mu_in, x) + G(mu_out, x) = G(mu_in + mu_out, x): one pass over x,
+    // with the input line kept in registers
+    double d0[S];
+#pragma unroll
+    for (int r = 0; r < S; ++r) d0[r] = d[r];
     explicit_step();
     thomas();
-    grad_line(x1);
+#pragma unroll
+    for (int r = 0; r < S; ++r) d0[r] += d[r];
+    grad_line(d0, x1);
+  } else {
+    grad_line(d, x1);
+    explicit_step();
+    thomas();
+    grad_line(d, x1);
   }
 
   // per-(line, level) gradient: segments summed in order by segment 0

@@ -184,8 +184,10 @@ int gw_adjoint(gw_ctx *ctx, const void *phi0, double t0, double dt, int32_t nste
  * for the conductivity gradient, discarded here); 2 transposed implicit +
  * explicit x (x: the forward implicit-x result); 3 transposed explicit y;
  * 4 transposed z-step R(dt A_z) (x: the forward field it acted on);
- * 5 lambda = W mu.  One FP64 scenario; synchronous; no counterpart in the
- * reference (it has no adjoint). */
+ * 5 lambda = W mu; 6 transposed z-step followed by explicit y (the pair
+ * gw_line_op mode 3 applies; x: the field the z-step acted on).  Ops 4 and 6
+ * need out distinct from in and x.  One FP64 scenario; synchronous; no
+ * counterpart in the reference (it has no adjoint). */
 int gw_adjoint_op(gw_ctx *ctx, int32_t op, void *out, const void *in, const void *x, double dt);
 
 /* Introspection (cache fields the reference exposes). */

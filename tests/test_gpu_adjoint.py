@@ -123,11 +123,11 @@ def _transpose_check(g, cache, fwd, op, lam, xin, xfield=None):
     assert abs(lhs - rhs) <= 1e-12 * scale, (op, lhs, rhs, scale)
 
 
-@pytest.mark.parametrize("op", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4, 6])
 def test_adjoint_operator_transposes(op):
     """Dot-product test of every piece of the reverse step against its forward
     operator through the C ABI (gw_advect, gw_implicit, gw_line_op,
-    gw_explicit, gw_zstep)."""
+    gw_explicit, gw_zstep, gw_line_op z-step + explicit y)."""
     g, cache, d, rnd = _dot_case()
     lib = cache._lib
     dt, c = 60.0, 30.0
@@ -154,9 +154,14 @@ def test_adjoint_operator_transposes(op):
             o = v.new_empty(v.shape)
             g._abi.check(lib.gw_explicit(cache.ctx, cache.check_state(o), cache.check_state(v), c, 1))
             return o
-    else:
+    elif op == 4:
         def fwd(v):
             o = v.clone()
             g._abi.check(lib.gw_zstep(cache.ctx, cache.check_state(o), dt))
             return o
-    _transpose_check(g, cache, fwd, op, lam, x, xfield=d if op in (1, 2, 4) else None)
+    else:
+        def fwd(v):
+            o = v.new_empty(v.shape)
+            g._abi.check(lib.gw_line_op(cache.ctx, cache.check_state(o), cache.check_state(v), c, dt, 1, 3))
+            return o
+    _transpose_check(g, cache, fwd, op, lam, x, xfield=d if op in (1, 2, 4, 6) else None)
