@@ -242,6 +242,7 @@ struct gw_ctx {
   // tile offsets of the line-length classes, per k-window family
   int64_t xcls[2][kLineClasses + 1] = {}, ycls[2][kLineClasses + 1] = {}, hcls[2][kLineClasses + 1] = {};
   int32_t hslots[2][kLineClasses] = {};
+  int64_t hone[2][kLineClasses] = {};
   // k-window width (64 or 32) of each line operator (mode, direction): the
   // length default until the first gw_steps times both widths (autotune), or
   // the width the problem flags force
@@ -288,7 +289,10 @@ struct gw_ctx {
       v.xtiles[f] = d_xtiles[f]; v.ytiles[f] = d_ytiles[f]; v.htiles[f] = d_htiles[f];
       for (int q = 0; q <= kLineClasses; ++q) {
         v.xcls[f][q] = xcls[f][q]; v.ycls[f][q] = ycls[f][q]; v.hcls[f][q] = hcls[f][q];
-        if (q < kLineClasses) v.hslots[f][q] = hslots[f][q];
+        if (q < kLineClasses) {
+          v.hslots[f][q] = hslots[f][q];
+          v.hone[f][q] = hone[f][q];
+        }
       }
     }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
@@ -916,14 +920,19 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       off[kLineClasses] = (int64_t)out.size();
     };
     for (int f = 0; f < 2; ++f) {
-      gather(xb[f], xt[f], ctx->xcls[f]);
-      gather(yb[f], yt[f], ctx->ycls[f]);
-      gather(hb[f], ht[f], ctx->hcls[f]);
+      // halo tiles of one set first within each class (launched with one slot)
       for (int q = 0; q < kLineClasses; ++q) {
+        const auto mid = std::stable_partition(hb[f][q].begin(), hb[f][q].end(),
+                                               [](const LineTile& t) { return t.nset == 1; });
         int m = 1;
         for (const LineTile& t : hb[f][q]) m = std::max(m, (int)t.nset);
         ctx->hslots[f][q] = m;
+        ctx->hone[f][q] = (int64_t)(mid - hb[f][q].begin());  // made absolute below
       }
+      gather(xb[f], xt[f], ctx->xcls[f]);
+      gather(yb[f], yt[f], ctx->ycls[f]);
+      gather(hb[f], ht[f], ctx->hcls[f]);
+      for (int q = 0; q < kLineClasses; ++q) ctx->hone[f][q] += ctx->hcls[f][q];
     }
     ctx->d_sc = ctx->up(sd);
     ctx->d_cell_prefix = ctx->up(cellp);
@@ -1040,9 +1049,10 @@ int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end
 }
 
 // Line-tile autotune: each line operator of the fused pipeline is timed in
-// both k-window widths, out of place from the caller's field into the second
-// buffer (the field is only read), and the faster width is kept for the
-// context.  Runs once, before the first step graph is captured.
+// both k-window widths, each with its natural LDS and capped at two tiles per
+// CU, out of place from the caller's field into the second buffer (the field
+// is only read), and the fastest is kept for the context.  Runs once, before
+// the first step graph is captured.
 int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
   static const int ops[4][2] = {{LINE_ZY, 1}, {LINE_IMPLICIT_EXPLICIT, 0}, {LINE_YZY, 1}, {LINE_IMPLICIT, 1}};
   const DevView v = ctx.view(tab);
@@ -1053,7 +1063,7 @@ int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
     const int mode = ops[q][0], dir = ops[q][1];
     float best = 0.f;
     int best_lw = ctx.lw_sel[mode][dir];
-    for (int lw : {64, 32}) {
+    for (int lw : {64, 32, 64 | kLwTwoTiles, 32 | kLwTwoTiles}) {
       auto run = [&]() {
         return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream),
                           "line tune");
@@ -1588,8 +1598,9 @@ int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void
       case 1:
       case 2: {
         const int mode = op == 1 ? ADJ_Y : ADJ_X;
-        const int lw = C.adj_lw[mode] ? C.adj_lw[mode] : (mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1)
-                                                                         : C.lw(LINE_IMPLICIT_EXPLICIT, 0));
+        const int lw = C.adj_lw[mode] ? C.adj_lw[mode]
+                                       : (mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1) : C.lw(LINE_IMPLICIT_EXPLICIT, 0)) &
+                                             ~kLwTwoTiles;
         rc = hip_status(lw == 32 ? line_adj_w32(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream)
                                  : line_adj_w64(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream),
                         "adjoint line");

@@ -35,6 +35,7 @@ struct DevView {
   const LineTile* htiles[2];    // y-line windows with a 3-lane k halo (fused z-step)
   int64_t xcls[2][kLineClasses + 1], ycls[2][kLineClasses + 1], hcls[2][kLineClasses + 1];
   int32_t hslots[2][kLineClasses];  // most coefficient sets of a halo tile of the class (LDS slots)
+  int64_t hone[2][kLineClasses];    // halo tiles of class q with one set: [hcls[q], hone[q]) (they come first)
   const uint16_t* cls;
   const int32_t* zone;
   const double* zcoef;          // pairs {ea, eb}
@@ -59,7 +60,15 @@ struct DevView {
   int32_t nx_max, ny_max, nz_max, countz_max;
   int64_t cells_total, cols_total;
   void* aux;                    // LINE_IMPLICIT_EXPLICIT_KEEP: receives the implicit result
+  int32_t lds_floor;            // bytes: each line tile requests at least this much LDS
 };
+
+// The k-window argument of launch_line is 64 or 32, optionally with
+// kLwTwoTiles set: every tile then requests kLdsTwoTiles bytes of LDS, so at
+// most two share a CU (some operators run faster with fewer concurrent tiles;
+// the autotune decides per operator).
+constexpr int kLwTwoTiles = 1;
+constexpr int kLdsTwoTiles = 80 * 1024;
 
 // LINE_IMPLICIT_EXPLICIT_KEEP is LINE_IMPLICIT_EXPLICIT that also writes the
 // implicit result to DevView::aux (FP64 only; the adjoint's recompute keeps

@@ -4,6 +4,7 @@
 // (gw_line_w<LW>_<prec>.hip).
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "gw_device.hpp"
@@ -86,8 +87,17 @@ GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
   return x * per + (x < rem ? x : rem) + i;
 }
 
+// Waves per SIMD the register budget aims at: a tile of NW waves puts NW/4 on
+// each SIMD, so four-wave tiles run three to a CU when the line fits in 168
+// registers (S <= 52), eight-wave tiles two to a CU when it fits in 128.
+template <class Cfg, int MODE>
+constexpr int line_min_waves() {
+  if (MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_ZY) return 2;
+  return Cfg::NW <= 4 ? (Cfg::S <= 52 ? 3 : 2) : (Cfg::S <= 32 ? 4 : 2);
+}
+
 template <class Cfg, class TS, int MODE>
-__global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS* out, const TS* in,
+__global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k_line(DevView v, int dir, TS* out, const TS* in,
                                                           double c, double a1, double a2, double a3,
                                                           int64_t tile0, int64_t ntiles) {
   using MT = ModeTraits<MODE>;
@@ -284,6 +294,13 @@ __global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS
     }
   }
 
+  // the solved line is complete here: without this the compiler interleaves
+  // the last fix-up pass with the next phase and holds both sets of values
+  if (MT::impl) {
+#pragma unroll
+    for (int r = 0; r < S; ++r) asm volatile("" : "+v"(d[r]));
+  }
+
   if (MT::keep && k >= T.ko0 && k < T.ko1) {
     TS* pk = static_cast<TS*>(v.aux) + useg;
 #pragma unroll
@@ -385,7 +402,7 @@ __global__ __launch_bounds__(Cfg::THREADS, 2) void k_line(DevView v, int dir, TS
 template <class Cfg, class TS, int MODE>
 static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
                         double a3, int64_t tile0, int64_t ntiles, int slots, hipStream_t st) {
-  const size_t sh = line_lds_doubles<Cfg, MODE>(slots) * sizeof(double);
+  const size_t sh = std::max(line_lds_doubles<Cfg, MODE>(slots) * sizeof(double), (size_t)v.lds_floor);
   hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
                      c, a1, a2, a3, tile0, ntiles);
 }
@@ -473,25 +490,30 @@ hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const vo
   const bool halo = mode == LINE_ZY || mode == LINE_YZY;
   const int f = LW == 32 ? 1 : 0;
   const int64_t* cls = halo ? v.hcls[f] : (dir == 0 ? v.xcls[f] : v.ycls[f]);
-  for (int q = 0; q < kLineClasses; ++q) {
-    const int64_t t0 = cls[q], n = cls[q + 1] - cls[q];
-    if (n == 0) continue;
-    const int hs = halo ? v.hslots[f][q] : kSlots;
-    hipError_t e;
-    switch (q) {
-      case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+  // halo tiles: those with one coefficient set first, launched with one LDS
+  // slot (most windows meet a single material zone), then the others
+  for (int q = 0; q < kLineClasses; ++q)
+    for (int r = 0; r < (halo ? 2 : 1); ++r) {
+      const int64_t t0 = (halo && r == 1) ? v.hone[f][q] : cls[q];
+      const int64_t t1 = (halo && r == 0) ? v.hone[f][q] : cls[q + 1];
+      const int64_t n = t1 - t0;
+      if (n == 0) continue;
+      const int hs = halo ? (r == 0 ? 1 : v.hslots[f][q]) : kSlots;
+      hipError_t e;
+      switch (q) {
+        case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+        default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
+      }
+      if (e != hipSuccess) return e;
     }
-    if (e != hipSuccess) return e;
-  }
   return hipSuccess;
 }
 

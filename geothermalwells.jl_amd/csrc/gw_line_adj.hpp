@@ -298,8 +298,16 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
     }
   };
 
+  // the line is complete after each solve: without the register barrier the
+  // compiler interleaves the solve's last pass with the next phase and holds
+  // both sets of values
+  auto settle = [&](double(&dd)[S]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < S; ++r) asm volatile("" : "+v"(dd[r]));
+  };
   if (MODE == ADJ_Y) {
     thomas();
+    settle(d);
     grad_line(d, x1);
   } else if (S <= 32) {
     // G(mu_in, x) + G(mu_out, x) = G(mu_in + mu_out, x): one pass over x,
@@ -309,6 +317,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
     for (int r = 0; r < S; ++r) d0[r] = d[r];
     explicit_step();
     thomas();
+    settle(d);
 #pragma unroll
     for (int r = 0; r < S; ++r) d0[r] += d[r];
     grad_line(d0, x1);
@@ -316,6 +325,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
     grad_line(d, x1);
     explicit_step();
     thomas();
+    settle(d);
     grad_line(d, x1);
   }
 
