@@ -51,22 +51,39 @@ PROFILE = os.path.join(ROOT, "profiles", "c1_3x3_fp64_step_kernels.md")
 
 
 def profile_traffic(mode):
-    """HBM bytes per launch of the steady-state line kernel of `mode` (the row
-    with the most calls) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload, with its kernel name and trace duration."""
+    """HBM bytes of one application of the line operator `mode` from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload: its
+    steady-state kernel (the k_line instance with the most calls) summed over
+    the launch ranges it is split into (one row per launch size), with the
+    kernel name and the summed trace duration.  None if the summary is absent
+    or not in the expected form."""
     if not os.path.exists(PROFILE):
         return None
-    best = None
-    for line in open(PROFILE):
-        cells = [c.strip() for c in line.split("|")]
-        if len(cells) < 9 or not cells[1].startswith("`void k_line<") or not cells[1].endswith(f"double, {mode}>`"):
-            continue
-        calls, avg_us, rd, wr = int(cells[2]), float(cells[3]), float(cells[5]), float(cells[6])
-        if best is None or calls > best[1]:
-            best = (cells[1].strip("`"), calls, avg_us, (rd + wr) * 2 ** 20)
-    if best is None:
+    try:
+        col = None
+        rows = {}
+        for line in open(PROFILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells and cells[0] == "kernel":
+                col = {name: q for q, name in enumerate(cells)}
+                continue
+            if col is None or len(cells) != len(col) or not cells[0].startswith("`void k_line<"):
+                continue
+            if not cells[0].endswith(f"double, {mode}>`"):
+                continue
+            name = cells[0].strip("`")
+            r = rows.setdefault(name, [0, 0.0, 0.0])
+            r[0] += int(cells[col["calls"]])
+            r[1] += float(cells[col["avg us"]])
+            r[2] += (float(cells[col["HBM read MiB (scaled)"]]) + float(cells[col["HBM write MiB (scaled)"]])) * 2 ** 20
+        if not rows:
+            return None
+        name = max(rows, key=lambda n: rows[n][0])
+        if not rows[name][2] > 0:
+            return None
+        return {"kernel": name, "trace_avg_us": rows[name][1], "bytes_per_launch": rows[name][2]}
+    except (ValueError, KeyError, OSError):
         return None
-    return {"kernel": best[0], "trace_avg_us": best[2], "bytes_per_launch": best[3]}
 
 
 def cpu_baseline(w, threads, steps):

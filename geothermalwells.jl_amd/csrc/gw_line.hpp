@@ -189,8 +189,10 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   const double* eb = cs + MT::A_EB * NM;
 
   // Every element loop below runs in chunks of KC elements: the LDS operands
-  // of the next chunk are fetched while the current one computes.
-  constexpr int KC = 4;
+  // of the next chunk are fetched while the current one computes.  The long
+  // FP64 segments of the fused z-step modes use chunks of two (their
+  // registers are otherwise short at three waves per SIMD).
+  constexpr int KC = (MT::halo && S > 40 && sizeof(TS) == 8) ? 2 : 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
   if (MT::impl) {
     // forward: y_l = b_l - fw_l y_{l-1}
@@ -312,44 +314,45 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
     // z-step R(dt A_z) across lanes (lane = k), column class per line element;
     // neighbours k-1 / k+1 come from lanes -1 / +1 by DPP wave shifts (the
     // window edge lanes keep their own value: they are halo, never output)
-    TS ean[KC], ebn[KC];
+    constexpr int KZ = KC;
+    TS ean[KZ], ebn[KZ];
 #pragma unroll
-    for (int q = 0; q < KC; ++q) {
+    for (int q = 0; q < KZ; ++q) {
       const int cl = shK[l0 + q];
       ean[q] = (TS)shZ[(cl * LW + kk) * 2];
       ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
     }
 #pragma unroll
-    for (int r0 = 0; r0 < S; r0 += KC) {
-      TS eac[KC], ebc[KC];
+    for (int r0 = 0; r0 < S; r0 += KZ) {
+      TS eac[KZ], ebc[KZ];
 #pragma unroll
-      for (int q = 0; q < KC; ++q) { eac[q] = ean[q]; ebc[q] = ebn[q]; }
-      if (r0 + KC < S) {
+      for (int q = 0; q < KZ; ++q) { eac[q] = ean[q]; ebc[q] = ebn[q]; }
+      if (r0 + KZ < S) {
 #pragma unroll
-        for (int q = 0; q < KC; ++q) {
-          const int cl = shK[l0 + r0 + KC + q];
+        for (int q = 0; q < KZ; ++q) {
+          const int cl = shK[l0 + r0 + KZ + q];
           ean[q] = (TS)shZ[(cl * LW + kk) * 2];
           ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
         }
       }
-      // the chunk's KC elements advance through the three stages together
-      // (KC independent chains per stage); the stage loop stays a real loop:
+      // the chunk's KZ elements advance through the three stages together
+      // (KZ independent chains per stage); the stage loop stays a real loop:
       // unrolled, the compiler interleaves chunks and the line spills
-      TS u0[KC], w[KC];
+      TS u0[KZ], w[KZ];
 #pragma unroll
-      for (int q = 0; q < KC; ++q) { u0[q] = (TS)d[r0 + q]; w[q] = u0[q]; }
+      for (int q = 0; q < KZ; ++q) { u0[q] = (TS)d[r0 + q]; w[q] = u0[q]; }
 #pragma unroll 1
       for (int st = 0; st < 3; ++st) {
         const TS al = st == 0 ? (TS)a1 : (st == 1 ? (TS)a2 : (TS)a3);
 #pragma unroll
-        for (int q = 0; q < KC; ++q) {
+        for (int q = 0; q < KZ; ++q) {
           const TS wm = wave_shr1(w[q]);
           const TS wp = wave_shl1(w[q]);
           w[q] = u0[q] + al * (eac[q] * (wp - w[q]) - ebc[q] * (w[q] - wm));
         }
       }
 #pragma unroll
-      for (int q = 0; q < KC; ++q) d[r0 + q] = (RT)w[q];
+      for (int q = 0; q < KZ; ++q) d[r0 + q] = (RT)w[q];
     }
   }
 
