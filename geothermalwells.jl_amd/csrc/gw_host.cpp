@@ -1050,9 +1050,10 @@ int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end
 
 // Line-tile autotune: each line operator of the fused pipeline is timed in
 // both k-window widths, each with its natural LDS and capped at two tiles per
-// CU, out of place from the caller's field into the second buffer (the field
-// is only read), and the fastest is kept for the context.  Runs once, before
-// the first step graph is captured.
+// CU, and for the fused z-step modes with the tiles split by set count and
+// whole (kLwHaloWhole), out of place from the caller's field into the second
+// buffer (the field is only read); the fastest is kept for the context.  Runs
+// once, before the first step graph is captured.
 int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
   static const int ops[4][2] = {{LINE_ZY, 1}, {LINE_IMPLICIT_EXPLICIT, 0}, {LINE_YZY, 1}, {LINE_IMPLICIT, 1}};
   const DevView v = ctx.view(tab);
@@ -1063,7 +1064,9 @@ int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
     const int mode = ops[q][0], dir = ops[q][1];
     float best = 0.f;
     int best_lw = ctx.lw_sel[mode][dir];
-    for (int lw : {64, 32, 64 | kLwTwoTiles, 32 | kLwTwoTiles}) {
+    const bool halo = mode == LINE_ZY || mode == LINE_YZY;
+    for (int var = 0; var < (halo ? 8 : 4); ++var) {
+      const int lw = (var & 1 ? 32 : 64) | (var & 2 ? kLwTwoTiles : 0) | (var & 4 ? kLwHaloWhole : 0);
       auto run = [&]() {
         return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream),
                           "line tune");
@@ -1600,7 +1603,7 @@ int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void
         const int mode = op == 1 ? ADJ_Y : ADJ_X;
         const int lw = C.adj_lw[mode] ? C.adj_lw[mode]
                                        : (mode == ADJ_Y ? C.lw(LINE_IMPLICIT, 1) : C.lw(LINE_IMPLICIT_EXPLICIT, 0)) &
-                                             ~kLwTwoTiles;
+                                             ~kLwFlags;
         rc = hip_status(lw == 32 ? line_adj_w32(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream)
                                  : line_adj_w64(v, ag, mode, o, i, xf, nullptr, c, 0.0, 0.0, 0.0, C.stream),
                         "adjoint line");

@@ -293,10 +293,11 @@ int line_window_default(int nmax) { return nmax <= kLineClassMax[8] ? 64 : 32; }
 hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, void* out, const void* in,
                        double c, double dt_z, double c3, hipStream_t st) {
   const double a1 = 2.0 * c3 * dt_z, a2 = 0.5 * dt_z, a3 = dt_z;
-  if (lw & kLwTwoTiles) {
+  if (lw & kLwFlags) {
     DevView v2 = v;
-    v2.lds_floor = kLdsTwoTiles;
-    return launch_line(v2, prec, mode, dir, lw & ~kLwTwoTiles, out, in, c, dt_z, c3, st);
+    if (lw & kLwTwoTiles) v2.lds_floor = kLdsTwoTiles;
+    if (lw & kLwHaloWhole) v2.halo_whole = 1;
+    return launch_line(v2, prec, mode, dir, lw & ~kLwFlags, out, in, c, dt_z, c3, st);
   }
   if (lw == 64)
     return prec == 0 ? line_w64_f64(v, mode, dir, out, in, c, a1, a2, a3, st)

@@ -61,13 +61,19 @@ struct DevView {
   int64_t cells_total, cols_total;
   void* aux;                    // LINE_IMPLICIT_EXPLICIT_KEEP: receives the implicit result
   int32_t lds_floor;            // bytes: each line tile requests at least this much LDS
+  int32_t halo_whole;           // 1: the fused z-step modes launch each class as one range
 };
 
 // The k-window argument of launch_line is 64 or 32, optionally with
 // kLwTwoTiles set: every tile then requests kLdsTwoTiles bytes of LDS, so at
-// most two share a CU (some operators run faster with fewer concurrent tiles;
-// the autotune decides per operator).
+// most two share a CU (some operators run faster with fewer concurrent
+// tiles); and kLwHaloWhole: the fused z-step modes launch each class's tiles
+// as one range with LDS for the class's largest set count, instead of two
+// ranges split by set count (one launch less, fewer tiles per CU).  The
+// autotune decides per operator.
 constexpr int kLwTwoTiles = 1;
+constexpr int kLwHaloWhole = 2;
+constexpr int kLwFlags = kLwTwoTiles | kLwHaloWhole;
 constexpr int kLdsTwoTiles = 80 * 1024;
 
 // LINE_IMPLICIT_EXPLICIT_KEEP is LINE_IMPLICIT_EXPLICIT that also writes the

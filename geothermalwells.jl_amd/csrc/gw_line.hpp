@@ -494,14 +494,16 @@ hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const vo
   const int f = LW == 32 ? 1 : 0;
   const int64_t* cls = halo ? v.hcls[f] : (dir == 0 ? v.xcls[f] : v.ycls[f]);
   // halo tiles: those with one coefficient set first, launched with one LDS
-  // slot (most windows meet a single material zone), then the others
+  // slot (most windows meet a single material zone), then the others; or
+  // (halo_whole) each class as one range with its largest set count
+  const bool split = halo && !v.halo_whole;
   for (int q = 0; q < kLineClasses; ++q)
-    for (int r = 0; r < (halo ? 2 : 1); ++r) {
-      const int64_t t0 = (halo && r == 1) ? v.hone[f][q] : cls[q];
-      const int64_t t1 = (halo && r == 0) ? v.hone[f][q] : cls[q + 1];
+    for (int r = 0; r < (split ? 2 : 1); ++r) {
+      const int64_t t0 = (split && r == 1) ? v.hone[f][q] : cls[q];
+      const int64_t t1 = (split && r == 0) ? v.hone[f][q] : cls[q + 1];
       const int64_t n = t1 - t0;
       if (n == 0) continue;
-      const int hs = halo ? (r == 0 ? 1 : v.hslots[f][q]) : kSlots;
+      const int hs = !halo ? kSlots : (split && r == 0 ? 1 : v.hslots[f][q]);
       hipError_t e;
       switch (q) {
         case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
