@@ -8,6 +8,13 @@ the steps as a fused pipeline (the implicit y of one step shares a kernel
 with the next step's z-step and explicit y), replayed as one captured
 hipGraph per step.
 
+Besides the headline line the JSON carries, as extra objects: `batched` (64
+ragged single-borehole scenarios one at a time in FP64 against all of them in
+one context in FP64 and FP32), `large_fp32` (the 5x5 array, 601 x 601 x 2002
+cells, FP32 state against FP64) and `adjoint` (the layered single borehole,
+121 x 121 x 602 cells: the 30-day outlet gradient's cost in forward steps and
+central-difference checks); each can be switched off by its `--*-steps 0`.
+
 Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits: this
 build measures one GPU.
 """
