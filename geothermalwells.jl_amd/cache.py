@@ -91,7 +91,7 @@ class Cache:
     advection sub-step."""
 
     def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                 line_window=None):
+                 line_window=None, line_two_tiles=False, line_halo_whole=False):
         self._scen = scenarios
         self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
         if self.backend.type != "cuda":
@@ -110,7 +110,11 @@ class Cache:
         prob.rock2_c3 = self.rock2_c3
         if line_window not in (None, 64, 32):
             raise ValueError("line_window is None (autotune), 64 or 32")
-        prob.flags = {None: 0, 64: _abi.GW_FLAG_LINE_W64, 32: _abi.GW_FLAG_LINE_W32}[line_window]
+        if (line_two_tiles or line_halo_whole) and line_window is None:
+            raise ValueError("line_two_tiles / line_halo_whole go with a forced line_window")
+        prob.flags = ({None: 0, 64: _abi.GW_FLAG_LINE_W64, 32: _abi.GW_FLAG_LINE_W32}[line_window] |
+                      (_abi.GW_FLAG_LINE_TWO_TILES if line_two_tiles else 0) |
+                      (_abi.GW_FLAG_HALO_WHOLE if line_halo_whole else 0))
         h = ctypes.c_void_p()
         _abi.check(lib.gw_create(ctypes.byref(prob), ctypes.byref(h)))
         self._ctx = h
@@ -202,12 +206,16 @@ class Cache:
 
 
 def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inlet_model, precision="f64",
-                 stream=None, rock2_c3=DEFAULT_ROCK2_C3, line_window=None):
+                 stream=None, rock2_c3=DEFAULT_ROCK2_C3, line_window=None, line_two_tiles=False,
+                 line_halo_whole=False):
     """create_cache (src/cache.jl:55-116).  `line_window` forces the k-window
-    width (64 or 32 lanes) of the line kernels; None lets the first multi-step
-    call time both and keep the faster."""
+    width (64 or 32 lanes) of the line kernels, and with it `line_two_tiles`
+    (LDS padded to two tiles per CU) and `line_halo_whole` (fused z-step tiles
+    of a class in one launch); None lets the first multi-step call time the
+    variants and keep the fastest.  Results do not depend on these."""
     return Cache([_Scenario(gridx, gridy, gridz, materials, boreholes, inlet_model)], backend=backend,
-                 precision=precision, stream=stream, rock2_c3=rock2_c3, line_window=line_window)
+                 precision=precision, stream=stream, rock2_c3=rock2_c3, line_window=line_window,
+                 line_two_tiles=line_two_tiles, line_halo_whole=line_halo_whole)
 
 
 def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,

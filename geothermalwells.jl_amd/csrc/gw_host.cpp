@@ -781,12 +781,17 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
   if (!p || !out) return fail(GW_EINVAL, "null argument");
   if (p->n_scen <= 0 || !p->scen) return fail(GW_EINVAL, "no scenarios");
   if (p->precision != GW_F64 && p->precision != GW_F32_STATE) return fail(GW_EINVAL, "unknown precision");
-  if ((p->flags & ~(GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32)) ||
-      (p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32)) == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32))
-    return fail(GW_EINVAL, "unknown or conflicting flags");
+  const int32_t width = p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32);
+  const int32_t variant = p->flags & (GW_FLAG_LINE_TWO_TILES | GW_FLAG_HALO_WHOLE);
+  if ((p->flags & ~(width | variant)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) || (variant && !width))
+    return fail(GW_EINVAL, "unknown or conflicting flags (the launch variants need a forced width)");
   std::unique_ptr<gw_ctx> ctx(new gw_ctx());
   ctx->prec = p->precision;
-  ctx->force_lw = (p->flags & GW_FLAG_LINE_W64) ? 64 : ((p->flags & GW_FLAG_LINE_W32) ? 32 : 0);
+  ctx->force_lw = width == GW_FLAG_LINE_W64 ? 64 : (width == GW_FLAG_LINE_W32 ? 32 : 0);
+  if (ctx->force_lw) {
+    if (variant & GW_FLAG_LINE_TWO_TILES) ctx->force_lw |= kLwTwoTiles;
+    if (variant & GW_FLAG_HALO_WHOLE) ctx->force_lw |= kLwHaloWhole;
+  }
   ctx->c3 = p->rock2_c3 > 0 ? p->rock2_c3 : 1.0 / 16.0;
   ctx->zstab = stability_bound(ctx->c3);
   std::unordered_map<std::string, int> smap;
@@ -1533,7 +1538,7 @@ int adj_recompute(gw_ctx& ctx, const CTab* tab, const double* u, double* e, doub
 int adj_tune(gw_ctx& ctx, const CTab* tab, const AdjGeo& ag, double* out, const double* in, const double* x) {
   if (ctx.adj_lw[0]) return GW_OK;
   if (ctx.force_lw) {
-    ctx.adj_lw[0] = ctx.adj_lw[1] = ctx.force_lw;
+    ctx.adj_lw[0] = ctx.adj_lw[1] = ctx.force_lw & ~kLwFlags;
     return GW_OK;
   }
   const DevView v = ctx.view(tab);

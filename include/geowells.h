@@ -38,9 +38,13 @@ typedef enum gw_status {
 enum { GW_F64 = 0, GW_F32_STATE = 1 };
 
 /* gw_problem.flags.  The line kernels come in two k-window widths (64 and 32
- * lanes); by default the first gw_steps call times both for each line
- * operator of the step and keeps the faster.  These flags force one width. */
-enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2 };
+ * lanes) and launch variants; by default the first gw_steps call times them
+ * for each line operator of the step and keeps the fastest.  GW_FLAG_LINE_W64
+ * / _W32 force one width; with one of them, GW_FLAG_LINE_TWO_TILES pads each
+ * tile's LDS so at most two share a CU, and GW_FLAG_HALO_WHOLE launches the
+ * fused z-step tiles of a line class as one range (default: split by their
+ * coefficient-set count).  Results do not depend on these choices. */
+enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2, GW_FLAG_LINE_TWO_TILES = 4, GW_FLAG_HALO_WHOLE = 8 };
 
 /* Inlet models (src/physics.jl:283-318). */
 enum {

@@ -210,3 +210,28 @@ def test_golden_example_gpu():
     assert np.isfinite(got).all() and got.min() > -50 and got.max() < 200
     for i, j in [(0, 0), (0, -1), (-1, 0), (-1, -1)]:
         assert np.linalg.norm(got[i, j] - u0[i, j]) <= 1e-5 * np.linalg.norm(u0[i, j])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("window", [64, 32])
+@pytest.mark.parametrize("two_tiles", [False, True])
+@pytest.mark.parametrize("halo_whole", [False, True])
+def test_line_launch_variants(window, two_tiles, halo_whole, precision):
+    """Every launch variant the autotune may choose (k-window width, LDS padded
+    to two tiles per CU, fused z-step tiles split by set count or whole) gives
+    the oracle's steps, on a case whose halo windows meet up to four sets."""
+    torch = _torch()
+    g = H.gw()
+    w = H.small_case(**CASES["layer_above_bottom"])
+    cache = g.create_cache(precision=precision, line_window=window, line_two_tiles=two_tiles,
+                           line_halo_whole=halo_whole,
+                           **{k: w[k] for k in ("gridx", "gridy", "gridz", "materials", "boreholes", "inlet_model")})
+    orc = H.oracle_for(w)
+    u0 = _perturbed(orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"]), seed=3)
+    dtype = torch.float64 if precision == "f64" else torch.float32
+    d = _dev(u0, dtype)
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(d), 0.0, 60.0, 3))
+    ref, hist, _ = orc.run(u0, 0.0, 60.0, 3, cache.rock2_c3)
+    tol = FIELD_TOL if precision == "f64" else 1e-5
+    assert H.rel_err(_host(d, orc.shape), ref) <= tol
+    assert np.abs(cache.T_outlet - hist[-1]).max() <= (OUTLET_TOL if precision == "f64" else 1e-3)
