@@ -1,7 +1,7 @@
 // gw_adjoint.hip -- adjoint kernels that hold no line in registers (FP64, one
-// scenario): the conductivity gradient of an operator along x or y, the
-// transposed z-step with its gradient, the transposed advection sub-step and
-// the lambda = W mu conversion.  mu = lambda / W, W = hx hy hz rho_c (the
+// scenario): the transposed explicit y and z-step with their conductivity
+// gradients (streaming along the y-lines), the transposed advection sub-step
+// and the lambda = W mu conversion.  mu = lambda / W, W = hx hy hz rho_c (the
 // transposed diffusion operators are W op W^-1, see AdjGeo).
 #include "gw_device.hpp"
 
@@ -9,7 +9,7 @@ namespace gw {
 
 constexpr int kStreamWaves = 4;   // waves per streaming block
 constexpr int kWalk = 16;         // elements per wave of k_adj_yz (fully unrolled: no loop back-edge)
-constexpr int kZcMaxStream = 16;  // column classes staged by k_adj_z (the adjoint requires at most this many)
+constexpr int kZcMaxStream = 16;  // column classes staged by k_adj_yz (the adjoint requires at most this many)
 
 GW_DEV double cell_w(const AdjGeo& ag, const DevView& v, const ScenDev& S, int i, int j, int k) {
   const int cl = v.cls[S.o_cls + (int64_t)i * S.ny + j];

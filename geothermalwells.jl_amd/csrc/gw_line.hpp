@@ -21,8 +21,9 @@ namespace gw {
 // coalesced.  The Thomas recurrences run segment-locally from zero carry and
 // are then corrected by the carried value times the running product of the
 // multipliers; the carries are a G-step scan in LDS.  The lanes of a window
-// use at most three coefficient sets (one per material zone met in z), staged
-// in LDS and padded with identity rows so the element loops have no branches.
+// use at most three coefficient sets (one per material zone met in z; four in
+// the halo windows of ZY / YZY), staged in LDS and padded with identity rows
+// so the element loops have no branches.
 //
 // Modes (the reference's operators on the same direction, src/solver.jl):
 //   EXPLICIT            u <- (I + cA) u                               (:34-58)
@@ -32,6 +33,8 @@ namespace gw {
 //                       explicit y (:23-31 under ROCK2, then :444)
 //   YZY                 u <- (I + cA_y) R(dt A_z) (I - cA_y)^-1 u      (:461, then
 //                       the next step's z-step and explicit y)
+//   IMPLICIT_EXPLICIT_KEEP   IMPLICIT_EXPLICIT that also writes the implicit
+//                       result to DevView::aux (the adjoint's recompute)
 // ZY / YZY run on y-lines with a 3-lane halo in k: the z-stencil is applied
 // three times across lanes (wave-wide DPP shifts), so only lanes 3..LW-4 of a
 // window are outputs; the halo lanes re-solve their own lines.  A shift that
