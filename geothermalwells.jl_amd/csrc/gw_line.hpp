@@ -138,36 +138,80 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
   const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
 
-  // coefficient sets of the window's lanes, identity rows beyond N
+  // coefficient sets of the window's lanes, identity rows beyond N; each
+  // set's values are all loaded before any is written (one round trip per set)
   const int32_t* sids = v.setid + (dir == 0 ? S_.o_xset : S_.o_yset) + (int64_t)T.line * S_.nzone;
   const int sid = sids[v.zone[S_.o_zone + kc]];
   const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : (sid == T.set[3] ? 3 : 0));
+  constexpr int NL = (NM + Cfg::THREADS - 1) / Cfg::THREADS;  // staged elements per thread and array
   for (int q = 0; q < T.nset; ++q) {
     const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : (q == 2 ? T.set[2] : T.set[3]));
     const double* g = v.coef + v.set_off[sq];
     double* dst = shC + q * MT::NA * NM;
-    for (int l = tid; l < NM; l += Cfg::THREADS) {
+    double cv[NL][5];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int l = tid + i * Cfg::THREADS;
       const bool ok = l < N;
       if (MT::impl) {
-        dst[MT::A_BD * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
-        dst[MT::A_FW * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
-        dst[MT::A_BU * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
+        cv[i][0] = ok ? g[SET_BD * N + l] : 1.0;
+        cv[i][1] = ok ? g[SET_FW * N + l] : 0.0;
+        cv[i][2] = ok ? g[SET_BU * N + l] : 0.0;
       }
       if (MT::expl) {
-        dst[MT::A_EA * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
-        dst[MT::A_EB * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
+        cv[i][3] = ok ? g[SET_EA * N + l] : 0.0;
+        cv[i][4] = ok ? g[SET_EB * N + l] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int l = tid + i * Cfg::THREADS;
+      if (l < NM) {
+        if (MT::impl) {
+          dst[MT::A_BD * NM + l] = cv[i][0];
+          dst[MT::A_FW * NM + l] = cv[i][1];
+          dst[MT::A_BU * NM + l] = cv[i][2];
+        }
+        if (MT::expl) {
+          dst[MT::A_EA * NM + l] = cv[i][3];
+          dst[MT::A_EB * NM + l] = cv[i][4];
+        }
       }
     }
   }
   if (MT::halo) {
     const double* zc = v.zcoef + 2 * S_.o_zcoef;
     const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
-    for (int q = wv; q < ncls; q += Cfg::NW) {
-      shZ[(q * LW + kk) * 2 + hh] = zc[2 * ((int64_t)q * nz + kc) + hh];
-      if (H == 1) shZ[(q * LW + kk) * 2 + 1] = zc[2 * ((int64_t)q * nz + kc) + 1];
+    // classes in batches of four per wave: loads first, then the writes
+    for (int q0 = wv; q0 < ncls; q0 += 4 * Cfg::NW) {
+      double z0[4], z1[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = q0 + i * Cfg::NW;
+        z0[i] = q < ncls ? zc[2 * ((int64_t)q * nz + kc) + hh] : 0.0;
+        if (H == 1) z1[i] = q < ncls ? zc[2 * ((int64_t)q * nz + kc) + 1] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int q = q0 + i * Cfg::NW;
+        if (q < ncls) {
+          shZ[(q * LW + kk) * 2 + hh] = z0[i];
+          if (H == 1) shZ[(q * LW + kk) * 2 + 1] = z1[i];
+        }
+      }
     }
     const uint16_t* clsl = v.cls + S_.o_cls + (int64_t)S_.ny * T.line;
-    for (int l = tid; l < NM; l += Cfg::THREADS) shK[l] = clsl[l < N ? l : N - 1];
+    uint16_t kv[NL];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int l = tid + i * Cfg::THREADS;
+      kv[i] = l < NM ? clsl[l < N ? l : N - 1] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int l = tid + i * Cfg::THREADS;
+      if (l < NM) shK[l] = kv[i];
+    }
   }
 
   RT d[S];
