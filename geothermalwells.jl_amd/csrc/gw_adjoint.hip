@@ -288,8 +288,13 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
     g0[r] = e_end[r] > e_beg[r] ? ap.ent[e_beg[r]] : AdjEnt{0, 0, 0.0};
     g1[r] = e_end[r] > e_beg[r] + 1 ? ap.ent[e_beg[r] + 1] : AdjEnt{0, 0, 0.0};
   }
+  // the group's column entries through LDS (a vector load of the table in
+  // the loop would wait for every access in flight)
+  __shared__ AdjCol scol[kAdvGroupMax];
+  if (t < Gp.ncol) scol[t] = ap.col[Gp.col0 + t];
+  __syncthreads();
   for (int c = 0; c < Gp.ncol; ++c) {
-    const AdjCol ac = ap.col[Gp.col0 + c];
+    const AdjCol ac = scol[c];
     double* p = mu + ac.off;
     const double* rho = ag.rho + (int64_t)ac.cls * S.nz;
     double wk[R];
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
       wk[r] = k < nlev ? ac.hxy * ag.hz[k] * rho[k] : 1.0;
       if (k < nlev) lout[k] = wk[r] * p[k];
     }
-    __syncthreads();
+    lds_barrier();  // distinct columns: only lout needs the barrier
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = t + 256 * r;
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
     // advection leaves alone but the outlet mean still reads
     if (sd != 0.0)
       for (int k = nlev + t; k < B.n_outlev; k += 256) p[k] += sd / (ac.hxy * ag.hz[k] * rho[k]);
-    __syncthreads();
+    lds_barrier();
   }
 }
 

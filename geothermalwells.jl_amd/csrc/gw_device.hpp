@@ -22,6 +22,17 @@ GW_DEV const T* lane_ptr(const T* p, unsigned byte_off) {
   return reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + byte_off);
 }
 
+// Workgroup barrier that waits for this wave's LDS operations only: its
+// global loads and stores stay in flight (__syncthreads' fence waits for
+// them all).  For LDS buffers reused between passes whose global accesses
+// never alias.
+GW_DEV void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt and expcnt at their maxima (no wait)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // Marks a wave-uniform 64-bit value as such (it then lives in SGPRs).
 GW_DEV int64_t uniform64(int64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);

@@ -572,7 +572,7 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   // each (a block walks its columns one after another)
   int64_t npipe = 0;
   for (int b = 0; b < ctx.nbh_total; ++b) npipe += ctx.bh_n_in[b] + ctx.bh_n_an[b];
-  const int kAdvGroup = (int)std::min<int64_t>(8, std::max<int64_t>(1, npipe / 2048));
+  const int kAdvGroup = (int)std::min<int64_t>(kAdvGroupMax, std::max<int64_t>(1, npipe / 2048));
   int gb = 0, mean_off = 0, poff = 0;
   for (size_t si = 0; si < ctx.sc.size(); ++si) {
     const Scen& s = ctx.sc[si];

@@ -179,80 +179,103 @@ __global__ __launch_bounds__(kFinThreads) void k_pipe_final(DevView v, double* j
   if (jacc && threadIdx.x == 0) jacc[blockIdx.x] += v.tout[blockIdx.x];
 }
 
-// One block per advection group (up to kAdvGroup pipe columns of one borehole
-// and one kind, inner or annulus): the departure entries of the thread's
-// levels (t + 256 r) are loaded once for the group; each column is staged in
-// LDS (double-buffered, the next column's loads in flight meanwhile) so the
-// update runs in place.
+// One block per advection group (up to kAdvGroupMax pipe columns of one
+// borehole and one kind, inner or annulus): the departure entries of the
+// thread's levels (t + 256 r) and the columns' offsets are loaded once for the
+// group; each column is staged in LDS (double-buffered, the next column's
+// loads in flight meanwhile) so the update runs in place.
 template <class TS, int R>
 __global__ __launch_bounds__(256) void k_pipe_advect(DevView v, TS* __restrict__ phi) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TS* colv = reinterpret_cast<TS*>(smem_raw);  // [2][countz_max]
+  TS* colv = reinterpret_cast<TS*>(smem_raw);  // [2][256 R]
   const RedChunk Gp = v.agroup[blockIdx.x];
   const BhDev B = v.bh[Gp.bh];
   const ScenDev S = v.sc[B.scen];
   const int nlev = S.countz;
+  if (nlev <= 0) return;
   const int t = threadIdx.x;
   const Dep* dp = v.dep + B.dep_off + Gp.part;  // entries (inner, annulus) per level
-  const double inlet = v.inlet[Gp.bh];
   const int moff = B.mean_off - B.klo;  // means index of turnaround level k: moff + k
-  int typ[R], kl[R], kr[R];
-  double a[R];
+  // Slot r of the thread is level t + 256 r, clamped into the column: a slot
+  // past the column's end repeats its last level (same departure, same
+  // value, same address), so every load, LDS write and store runs without a
+  // branch and the compiler can count the stores in flight.  Per slot: the
+  // departure kind, the interpolation of an own departure and the value of a
+  // turnaround or inlet departure (the same for every column of the group).
+  int kc[R], typ[R], kl[R], kr[R];
+  TS w[R], cval[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int k = t + 256 * r;
-    if (k < nlev) {
-      const Dep e = dp[2 * k];
-      typ[r] = e.type; kl[r] = e.kl; kr[r] = e.kr; a[r] = e.a;
-    } else {
-      typ[r] = DEP_KEEP; kl[r] = 0; kr[r] = 0; a[r] = 0.0;
+    kc[r] = t + 256 * r < nlev ? t + 256 * r : nlev - 1;
+    const Dep e = dp[2 * kc[r]];
+    typ[r] = e.type; kl[r] = 0; kr[r] = 0; w[r] = (TS)0; cval[r] = (TS)0;
+    if (e.type == DEP_OWN) {
+      kl[r] = e.kl; kr[r] = e.kr; w[r] = (TS)e.a;
+    } else if (e.type == DEP_TURN) {
+      cval[r] = (TS)((1.0 - e.a) * v.means[moff + e.kl] + e.a * v.means[moff + e.kr]);
+    } else if (e.type == DEP_INLET) {
+      cval[r] = (TS)v.inlet[Gp.bh];
     }
   }
-  auto colp = [&](int c) {
-    const PipeCol pc = v.pcol[Gp.col0 + c];
-    return phi + S.off + ((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz;
+  // column offsets through LDS: a vector load of the column table inside the
+  // loop would wait for every load and store in flight (the table could alias
+  // phi as far as the compiler knows)
+  __shared__ int64_t coff[kAdvGroupMax];
+  if (t < Gp.ncol) {
+    const PipeCol pc = v.pcol[Gp.col0 + t];
+    coff[t] = S.off + ((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz;
+  }
+  __syncthreads();
+  auto colp = [&](int c) { return phi + uniform64(coff[c]); };
+  const int ncol = Gp.ncol;
+  // Columns go in pairs: both of a pair are staged in the two LDS buffers
+  // while the next pair's loads are in flight (a load index past the group
+  // repeats its last column; such values are never written back).
+  auto load_col = [&](TS (&dst)[R], int c) {
+    const TS* pn = colp(c < ncol ? c : ncol - 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[r] = pn[kc[r]];
   };
-  TS cur[R];
-  {
-    const TS* p = colp(0);
+  auto stage = [&](TS* buf, const TS (&cur)[R]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int k = t + 256 * r;
-      cur[r] = k < nlev ? p[k] : (TS)0;
-    }
-  }
-  for (int c = 0; c < Gp.ncol; ++c) {
+    for (int r = 0; r < R; ++r) buf[t + 256 * r] = cur[r];
+  };
+  auto update = [&](int c, const TS* buf) {
     TS* p = colp(c);
-    TS* buf = colv + (c & 1) * v.countz_max;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int k = t + 256 * r;
-      if (k < nlev) buf[k] = cur[r];
+      const TS own = ((TS)1 - w[r]) * buf[kl[r]] + w[r] * buf[kr[r]];
+      const TS o = typ[r] == DEP_OWN ? own : (typ[r] == DEP_KEEP ? buf[t + 256 * r] : cval[r]);
+      p[kc[r]] = o;  // a kept level is written back unchanged
     }
-    if (c + 1 < Gp.ncol) {
-      const TS* pn = colp(c + 1);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int k = t + 256 * r;
-        cur[r] = k < nlev ? pn[k] : (TS)0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int k = t + 256 * r;
-      if (typ[r] == DEP_KEEP) continue;
-      TS o;
-      if (typ[r] == DEP_OWN) {
-        const TS w = (TS)a[r];
-        o = ((TS)1 - w) * buf[kl[r]] + w * buf[kr[r]];
-      } else if (typ[r] == DEP_TURN) {
-        o = (TS)((1.0 - a[r]) * v.means[moff + kl[r]] + a[r] * v.means[moff + kr[r]]);
-      } else {
-        o = (TS)inlet;
-      }
-      p[k] = o;
-    }
+  };
+  TS* buf0 = colv;
+  TS* buf1 = colv + 256 * R;
+  TS ca[R], cb[R];
+  load_col(ca, 0);
+  load_col(cb, 1);
+  // the columns of a group are distinct, so only the LDS buffers need the
+  // barriers: the next pair's loads and this pair's stores stay in flight
+  auto pair = [&](int c) {
+    stage(buf0, ca);
+    stage(buf1, cb);
+    load_col(ca, c + 2);
+    load_col(cb, c + 3);
+    lds_barrier();
+    update(c, buf0);
+    update(c + 1, buf1);
+    lds_barrier();
+  };
+  if (ncol >= 2) {
+    // the first pair outside the loop: every loop entry then has the same
+    // accesses in flight and the loop waits for the staged pair's loads only
+    pair(0);
+    for (int c = 2; c + 1 < ncol; c += 2) pair(c);
+  }
+  if (ncol & 1) {
+    stage(buf0, ca);  // the last column, loaded as the first of a pair
+    lds_barrier();
+    update(ncol - 1, buf0);
   }
 }
 
@@ -323,15 +346,16 @@ hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipSt
 template <class TS>
 static hipError_t advect_t(const DevView& v, TS* phi, hipStream_t st) {
   const dim3 grid((unsigned)v.nagroup), block(256);
-  const size_t sh = 2 * (size_t)v.countz_max * sizeof(TS);
   const int R = (v.countz_max + 255) / 256;
-  if (R <= 1) hipLaunchKernelGGL((k_pipe_advect<TS, 1>), grid, block, sh, st, v, phi);
-  else if (R <= 2) hipLaunchKernelGGL((k_pipe_advect<TS, 2>), grid, block, sh, st, v, phi);
-  else if (R <= 3) hipLaunchKernelGGL((k_pipe_advect<TS, 3>), grid, block, sh, st, v, phi);
-  else if (R <= 4) hipLaunchKernelGGL((k_pipe_advect<TS, 4>), grid, block, sh, st, v, phi);
-  else if (R <= 6) hipLaunchKernelGGL((k_pipe_advect<TS, 6>), grid, block, sh, st, v, phi);
-  else if (R <= 8) hipLaunchKernelGGL((k_pipe_advect<TS, 8>), grid, block, sh, st, v, phi);
-  else if (R <= 16) hipLaunchKernelGGL((k_pipe_advect<TS, 16>), grid, block, sh, st, v, phi);
+  // two column buffers of 256 R levels (every thread's slots)
+  auto sh = [](int r) { return 2 * (size_t)256 * r * sizeof(TS); };
+  if (R <= 1) hipLaunchKernelGGL((k_pipe_advect<TS, 1>), grid, block, sh(1), st, v, phi);
+  else if (R <= 2) hipLaunchKernelGGL((k_pipe_advect<TS, 2>), grid, block, sh(2), st, v, phi);
+  else if (R <= 3) hipLaunchKernelGGL((k_pipe_advect<TS, 3>), grid, block, sh(3), st, v, phi);
+  else if (R <= 4) hipLaunchKernelGGL((k_pipe_advect<TS, 4>), grid, block, sh(4), st, v, phi);
+  else if (R <= 6) hipLaunchKernelGGL((k_pipe_advect<TS, 6>), grid, block, sh(6), st, v, phi);
+  else if (R <= 8) hipLaunchKernelGGL((k_pipe_advect<TS, 8>), grid, block, sh(8), st, v, phi);
+  else if (R <= 16) hipLaunchKernelGGL((k_pipe_advect<TS, 16>), grid, block, sh(16), st, v, phi);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

@@ -21,6 +21,7 @@ inline int line_class(int n) {
 // Pipe columns per reduction chunk (one partial sum per level).  Fixed per
 // borehole, so the summation order does not depend on the batch.
 constexpr int kRedChunk = 16;
+constexpr int kAdvGroupMax = 8;  // most pipe columns of an advection group
 
 struct DevView {
   const ScenDev* sc;
