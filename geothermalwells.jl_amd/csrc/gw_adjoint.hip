@@ -209,11 +209,14 @@ __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPi
   const ScenDev S = v.sc[B.scen];
   const int k0 = R.part == 0 ? A.t0 : A.i0;
   const int nl = R.part == 0 ? A.nt : A.ni;
+  __shared__ AdjCol scol[kRedChunk];  // the chunk's column entries, loaded once
+  if ((int)threadIdx.x < R.ncol) scol[threadIdx.x] = ap.col[R.col0 + threadIdx.x];
+  __syncthreads();
   for (int t = threadIdx.x; t < nl; t += blockDim.x) {
     const int lev = k0 + t;
     const double hz = ag.hz[lev];
     ap.partial[R.poff + t] = ordered_sum_adj<kRedChunk>(R.ncol, [&](int c) {
-      const AdjCol ac = ap.col[R.col0 + c];
+      const AdjCol ac = scol[c];
       return ac.hxy * hz * ag.rho[(int64_t)ac.cls * S.nz + lev] * mu[ac.off + lev];
     });
   }

@@ -164,11 +164,16 @@ __global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __res
   const int k0 = R.part == 0 ? 0 : B.klo;
   const int nl = R.part == 0 ? B.n_outlev : B.nlev;
   const TS* base = phi + S.off + k0;
+  // the chunk's column offsets once, through LDS (in the level loop each
+  // column would cost a table load before its value load)
+  __shared__ int64_t coff[kRedChunk];
+  if ((int)threadIdx.x < R.ncol) {
+    const PipeCol pc = v.pcol[R.col0 + threadIdx.x];
+    coff[threadIdx.x] = ((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz;
+  }
+  __syncthreads();
   for (int lev = threadIdx.x; lev < nl; lev += blockDim.x)
-    v.partial[R.poff + lev] = ordered_sum<kRedChunk>(R.ncol, [&](int c) {
-      const PipeCol pc = v.pcol[R.col0 + c];
-      return (double)base[((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz + lev];
-    });
+    v.partial[R.poff + lev] = ordered_sum<kRedChunk>(R.ncol, [&](int c) { return (double)base[coff[c] + lev]; });
 }
 
 // Pass 2: one block per borehole; jacc (may be NULL) accumulates the outlet
