@@ -61,9 +61,10 @@ def profile_traffic(mode):
     """HBM bytes of one application of the line operator `mode` from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload: its
     steady-state kernel (the k_line instance with the most calls) summed over
-    the launch ranges it is split into (one row per launch size), with the
-    kernel name and the summed trace duration.  None if the summary is absent
-    or not in the expected form."""
+    the launch ranges it is split into (its rows with that call count, one per
+    launch size; rows with fewer calls are the autotuner's timing launches of
+    other variants), with the kernel name and the summed trace duration.  None
+    if the summary is absent or not in the expected form."""
     if not os.path.exists(PROFILE):
         return None
     try:
@@ -79,16 +80,17 @@ def profile_traffic(mode):
             if not cells[0].endswith(f"double, {mode}>`"):
                 continue
             name = cells[0].strip("`")
-            r = rows.setdefault(name, [0, 0.0, 0.0])
-            r[0] += int(cells[col["calls"]])
-            r[1] += float(cells[col["avg us"]])
-            r[2] += (float(cells[col["HBM read MiB (scaled)"]]) + float(cells[col["HBM write MiB (scaled)"]])) * 2 ** 20
+            b = (float(cells[col["HBM read MiB (scaled)"]]) + float(cells[col["HBM write MiB (scaled)"]])) * 2 ** 20
+            rows.setdefault(name, []).append((int(cells[col["calls"]]), float(cells[col["avg us"]]), b))
         if not rows:
             return None
-        name = max(rows, key=lambda n: rows[n][0])
-        if not rows[name][2] > 0:
+        name = max(rows, key=lambda n: max(r[0] for r in rows[n]))
+        top = max(r[0] for r in rows[name])
+        ranges = [r for r in rows[name] if r[0] == top]
+        nbytes = sum(r[2] for r in ranges)
+        if not nbytes > 0:
             return None
-        return {"kernel": name, "trace_avg_us": rows[name][1], "bytes_per_launch": rows[name][2]}
+        return {"kernel": name, "trace_avg_us": sum(r[1] for r in ranges), "bytes_per_launch": nbytes}
     except (ValueError, KeyError, OSError):
         return None
 

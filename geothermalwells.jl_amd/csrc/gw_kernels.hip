@@ -207,20 +207,15 @@ __global__ __launch_bounds__(256) void k_pipe_advect(DevView v, TS* __restrict__
   // branch and the compiler can count the stores in flight.  Per slot: the
   // departure kind, the interpolation of an own departure and the value of a
   // turnaround or inlet departure (the same for every column of the group).
-  int kc[R], typ[R], kl[R], kr[R];
-  TS w[R], cval[R];
+  // Load order: departure entries, then (one barrier) the column offsets,
+  // then the first column, then the turnaround means the entries point at,
+  // so that no wait holds back a load that does not depend on it.
+  int kc[R];
+  Dep e[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     kc[r] = t + 256 * r < nlev ? t + 256 * r : nlev - 1;
-    const Dep e = dp[2 * kc[r]];
-    typ[r] = e.type; kl[r] = 0; kr[r] = 0; w[r] = (TS)0; cval[r] = (TS)0;
-    if (e.type == DEP_OWN) {
-      kl[r] = e.kl; kr[r] = e.kr; w[r] = (TS)e.a;
-    } else if (e.type == DEP_TURN) {
-      cval[r] = (TS)((1.0 - e.a) * v.means[moff + e.kl] + e.a * v.means[moff + e.kr]);
-    } else if (e.type == DEP_INLET) {
-      cval[r] = (TS)v.inlet[Gp.bh];
-    }
+    e[r] = dp[2 * kc[r]];
   }
   // column offsets through LDS: a vector load of the column table inside the
   // loop would wait for every load and store in flight (the table could alias
@@ -233,6 +228,30 @@ __global__ __launch_bounds__(256) void k_pipe_advect(DevView v, TS* __restrict__
   __syncthreads();
   auto colp = [&](int c) { return phi + uniform64(coff[c]); };
   const int ncol = Gp.ncol;
+  TS ca[R], cb[R];
+  {
+    const TS* p0 = colp(0);
+    const TS* p1 = colp(ncol >= 2 ? 1 : 0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) ca[r] = p0[kc[r]];
+    if (ncol >= 2) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) cb[r] = p1[kc[r]];
+    }
+  }
+  int typ[R], kl[R], kr[R];
+  TS w[R], cval[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    typ[r] = e[r].type; kl[r] = 0; kr[r] = 0; w[r] = (TS)0; cval[r] = (TS)0;
+    if (e[r].type == DEP_OWN) {
+      kl[r] = e[r].kl; kr[r] = e[r].kr; w[r] = (TS)e[r].a;
+    } else if (e[r].type == DEP_TURN) {
+      cval[r] = (TS)((1.0 - e[r].a) * v.means[moff + e[r].kl] + e[r].a * v.means[moff + e[r].kr]);
+    } else if (e[r].type == DEP_INLET) {
+      cval[r] = (TS)v.inlet[Gp.bh];
+    }
+  }
   // Columns go in pairs: both of a pair are staged in the two LDS buffers
   // while the next pair's loads are in flight (a load index past the group
   // repeats its last column; such values are never written back).
@@ -256,9 +275,6 @@ __global__ __launch_bounds__(256) void k_pipe_advect(DevView v, TS* __restrict__
   };
   TS* buf0 = colv;
   TS* buf1 = colv + 256 * R;
-  TS ca[R], cb[R];
-  load_col(ca, 0);
-  load_col(cb, 1);
   // the columns of a group are distinct, so only the LDS buffers need the
   // barriers: the next pair's loads and this pair's stores stay in flight
   auto pair = [&](int c) {
