@@ -14,6 +14,8 @@ one context in FP64 and FP32), `large_fp32` (the 5x5 array, 601 x 601 x 2002
 cells, FP32 state against FP64) and `adjoint` (the layered single borehole,
 121 x 121 x 602 cells: the 30-day outlet gradient's cost in forward steps and
 central-difference checks); each can be switched off by its `--*-steps 0`.
+`roofline` also gives the rate of a plain device copy of the field measured
+on the same box (`copy_GBps`) and the dominant kernel's fraction of it.
 
 Prints one JSON line (see the keys below).  `--gpus N` with N > 1 exits: this
 build measures one GPU.
@@ -93,6 +95,26 @@ def profile_traffic(mode):
         return {"kernel": name, "trace_avg_us": sum(r[1] for r in ranges), "bytes_per_launch": nbytes}
     except (ValueError, KeyError, OSError):
         return None
+
+
+def copy_rate(u, reps=5):
+    """Bytes per second (read + write) of a plain device copy of the field
+    on this box: the practical ceiling of a kernel that reads and writes the
+    field once."""
+    import torch
+    tmp = torch.empty_like(u)
+    tmp.copy_(u)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        tmp.copy_(u)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del tmp
+    return 2 * u.numel() * u.element_size() / best
 
 
 def cpu_baseline(w, threads, steps):
@@ -374,6 +396,7 @@ def main():
     dom_bytes = cells * 2 * s  # one read + one write of the state per launch
     achieved = dom_bytes / (per_op[dom] * 1e-3)
     traffic = profile_traffic({"implicit_explicit_x": 2, "yzy": 4}[dom])
+    copy_bw = copy_rate(u)
     B_cs = 10 * s
     Bp = pipe_bytes(cache, w, s)
     steps_per_s = args.steps / wall
@@ -400,7 +423,8 @@ def main():
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_source": (f"{os.path.relpath(PROFILE, ROOT)}: {traffic['kernel']} "
                                         f"(trace avg {traffic['trace_avg_us']:.1f} us)") if traffic else None,
-                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": per_op[dom]},
+                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_ms": per_op[dom],
+                     "copy_GBps": copy_bw / 1e9, "frac_of_copy": achieved / copy_bw},
         "step_roofline": {"B_cs_5sweep": B_cs, "pipe_bytes_per_step": Bp,
                           "frac_5sweep": step_bw / HBM_PEAK, "frac_2sweep_as_run": fused_bw / HBM_PEAK},
         "per_op_ms": per_op,

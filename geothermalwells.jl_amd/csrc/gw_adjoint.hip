@@ -59,11 +59,34 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   const int k = T.kw + lane;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
   const int ncls = S.ncls < kZcMaxStream ? S.ncls : kZcMaxStream;
-  for (int q = w; q < ncls; q += kStreamWaves) {
-    const double* zc = v.zcoef + 2 * (S.o_zcoef + (int64_t)q * nz + kc);
-    shZ[(q * 64 + lane) * 2] = zc[0];
-    shZ[(q * 64 + lane) * 2 + 1] = zc[1];
-    shR[q * 64 + lane] = ag.rock[(int64_t)q * nz + kc];
+  // the lane's explicit-y set (zone -> set id -> set offset, three dependent
+  // loads) starts before the z-coefficient staging so the two overlap
+  int64_t so = 0;
+  if (EY) so = v.set_off[v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]]];
+  // z coefficients and rock terms of the column classes: all of a wave's loads
+  // before its LDS writes (one round trip)
+  {
+    constexpr int QW = kZcMaxStream / kStreamWaves;
+    double z0[QW], z1[QW], zr[QW];
+#pragma unroll
+    for (int s = 0; s < QW; ++s) {
+      const int q = w + s * kStreamWaves;
+      const bool ok = q < ncls;
+      const int qc = ok ? q : 0;
+      const double* zc = v.zcoef + 2 * (S.o_zcoef + (int64_t)qc * nz + kc);
+      z0[s] = ok ? zc[0] : 0.0;
+      z1[s] = ok ? zc[1] : 0.0;
+      zr[s] = ok ? ag.rock[(int64_t)qc * nz + kc] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < QW; ++s) {
+      const int q = w + s * kStreamWaves;
+      if (q < ncls) {
+        shZ[(q * 64 + lane) * 2] = z0[s];
+        shZ[(q * 64 + lane) * 2 + 1] = z1[s];
+        shR[q * 64 + lane] = zr[s];
+      }
+    }
   }
   __syncthreads();
   const double izk = (k >= 0 && k < nz - 1) ? ag.iz[k] : 0.0;
@@ -75,8 +98,6 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   // constant 32-bit lane offset per load)
   unsigned ofa = 0, ofb = 0;
   if (EY) {
-    const int sid = v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]];
-    const int64_t so = v.set_off[sid];
     ofa = (unsigned)((so + SET_EA * (int64_t)N) * (int64_t)sizeof(double));
     ofb = (unsigned)((so + SET_EB * (int64_t)N) * (int64_t)sizeof(double));
   }
