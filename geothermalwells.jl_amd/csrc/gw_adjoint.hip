@@ -46,9 +46,18 @@ template <bool EY>
 __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo ag, double* out, const double* mu,
                                                              const double* u, double c, double a1, double a2,
                                                              double a3) {
-  __shared__ double shZ[kZcMaxStream * 64 * 2];
-  __shared__ double shR[kZcMaxStream * 64];
+  // z coefficients [ncls][64][2] and rock terms [ncls][64] of the scenario's
+  // column classes: launched with LDS for its class count only
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ncls_l = ag.ncls < kZcMaxStream ? ag.ncls : kZcMaxStream;
+  double* shZ = reinterpret_cast<double*>(smem_raw);
+  double* shR = shZ + ncls_l * 64 * 2;
   __shared__ double sh[64 * kStreamWaves];
+  // per-element values that are the same for every lane (hy_j, iy_{j-1} and
+  // the column class), for the block's elements and their neighbours
+  constexpr int kJs = kStreamWaves * kWalk + 4;
+  __shared__ double shHy[kJs], shIy[kJs];
+  __shared__ int shCl[kJs];
   const int parts = ag.yparts;
   const int64_t tb = blockIdx.x / parts;
   const int p = (int)(blockIdx.x - tb * parts);
@@ -58,11 +67,19 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   const int N = S.ny, nz = S.nz, i = T.line;
   const int k = T.kw + lane;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
-  const int ncls = S.ncls < kZcMaxStream ? S.ncls : kZcMaxStream;
+  const int ncls = ncls_l;
   // the lane's explicit-y set (zone -> set id -> set offset, three dependent
   // loads) starts before the z-coefficient staging so the two overlap
   int64_t so = 0;
   if (EY) so = v.set_off[v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]]];
+  // block's elements j (clamped) from jlo: la - 1 .. lb + 2 of its waves
+  const int jlo = max(0, p * kStreamWaves * kWalk - 1);
+  for (int q = threadIdx.x; q < kJs; q += 64 * kStreamWaves) {
+    const int jj = min(jlo + q, N - 1);
+    shHy[q] = ag.hy[jj];
+    shIy[q] = ag.iy[jj > 0 ? jj - 1 : 0];
+    shCl[q] = v.cls[S.o_cls + (int64_t)i * S.ny + jj];
+  }
   // z coefficients and rock terms of the column classes: all of a wave's loads
   // before its LDS writes (one round trip)
   {
@@ -108,7 +125,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   const int64_t cbase = uniform64(S.off + (int64_t)i * S.ny * nz);
   const unsigned kcb = (unsigned)kc * (unsigned)sizeof(double);
   auto at = [&](const double* f, int j) { return *lane_ptr(f + cbase + (int64_t)j * nz, kcb); };
-  auto cls_at = [&](int j) { return __builtin_amdgcn_readfirstlane((int)v.cls[S.o_cls + (int64_t)i * S.ny + j]); };
+  auto cls_at = [&](int j) { return __builtin_amdgcn_readfirstlane(shCl[j - jlo]); };
   // a value with its lane neighbours (levels k+1, k-1): each shifted once and
   // shared by the z operator and the face products (the kernel is bound by
   // instruction issue, and a 64-bit lane shift is two DPP moves)
@@ -140,17 +157,13 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
     // elements ahead (the explicit step needs mu_{j+1}), u and the
     // coefficients two ahead
     auto jc = [&](int j) { return j < N ? j : N - 1; };
-    double rm[4], ru[4], rea[4], reb[4], rhy[4], riy[4];
-    int rcl[4];
+    double rm[4], ru[4], rea[4], reb[4];
     auto load_mu = [&](int slot, int j) { rm[slot & 3] = at(mu, jc(j)); };
     auto load_rest = [&](int slot, int j) {
       const int jj = jc(j);
       ru[slot & 3] = at(u, jj);
       rea[slot & 3] = EY ? *lane_ptr(v.coef + jj, ofa) : 0.0;
       reb[slot & 3] = EY ? *lane_ptr(v.coef + jj, ofb) : 0.0;
-      rcl[slot & 3] = cls_at(jj);
-      rhy[slot & 3] = ag.hy[jj];
-      riy[slot & 3] = ag.iy[jc(j > 0 ? j - 1 : 0)];  // used for j > 0 only
     };
 #pragma unroll
     for (int q = 0; q < 3; ++q) load_mu(q, la + q);
@@ -160,7 +173,8 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
       load_mu(s + 3, j + 3);
       load_rest(s + 2, j + 2);
       const double m_cur = rm[s & 3], m_nx = rm[(s + 1) & 3], u_cur = ru[s & 3];
-      const int cl = rcl[s & 3];
+      const int jj = jc(j);
+      const int cl = cls_at(jj);
       const double ez = shZ[(cl * 64 + lane) * 2], fz = shZ[(cl * 64 + lane) * 2 + 1];
       const double rk = shR[cl * 64 + lane];
       auto Az = [&](const Nb& q) { return ez * (q.p - q.c) - fz * (q.c - q.m); };
@@ -170,7 +184,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
       double m = m_cur;
       if (EY) {
         const double z = u_cur + a3 * Az(n2);
-        const double fy = riy[s & 3] * (m_cur - m_prev) * (z - z_prev) * (r_prev + rk);
+        const double fy = shIy[jj - jlo] * (m_cur - m_prev) * (z - z_prev) * (r_prev + rk);  // used for j > 0 only
         gy += (act && j > 0) ? fy : 0.0;
         m = m_cur + c * (rea[s & 3] * (m_nx - m_cur) - reb[s & 3] * (m_cur - m_prev));
         z_prev = z;
@@ -181,7 +195,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
       const Nb nm1 = nb(a2 * Az(nm2));
       const double o = m + nm2.c + nm1.c + a1 * Az(nm1);
       const double fs = a3 * face(nm, n2) + a2 * face(nm2, n1) + a1 * face(nm1, nu);
-      const double vup = -hxi * rhy[s & 3] * izk * fs;
+      const double vup = -hxi * shHy[jj - jlo] * izk * fs;
       const double fz2 = 0.5 * rk * (vup + wave_shr1(vup));
       gz += act ? fz2 : 0.0;
       // the sums are completed here: otherwise the compiler defers every
@@ -365,8 +379,9 @@ hipError_t launch_adj_yz(const DevView& v, const AdjGeo& ag, bool ey, double* ou
   if (n == 0) return hipSuccess;
   const double a1 = 2.0 * c3 * dt, a2 = 0.5 * dt, a3 = dt;
   const dim3 grid((unsigned)(n * ag.yparts)), block(64 * kStreamWaves);
-  if (ey) hipLaunchKernelGGL(k_adj_yz<true>, grid, block, 0, st, v, ag, out, mu, u, c, a1, a2, a3);
-  else hipLaunchKernelGGL(k_adj_yz<false>, grid, block, 0, st, v, ag, out, mu, u, c, a1, a2, a3);
+  const size_t sh = (size_t)std::min(ag.ncls, kZcMaxStream) * 64 * 3 * sizeof(double);
+  if (ey) hipLaunchKernelGGL(k_adj_yz<true>, grid, block, sh, st, v, ag, out, mu, u, c, a1, a2, a3);
+  else hipLaunchKernelGGL(k_adj_yz<false>, grid, block, sh, st, v, ag, out, mu, u, c, a1, a2, a3);
   return hipGetLastError();
 }
 

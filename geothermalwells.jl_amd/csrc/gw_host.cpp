@@ -1399,6 +1399,7 @@ AdjGeo make_adj_geo(const gw_ctx& ctx, DevBufs& db) {
   ag.rho = db.up(s.rprof);
   ag.rock = db.up(rock);
   ag.yparts = adj_yparts(ctx.hcls[0][kLineClasses], s.ny);
+  ag.ncls = s.ncls;
   ag.gx_part = db.up(std::vector<double>((size_t)s.ny * s.nz, 0.0));
   ag.gy_part = db.up(std::vector<double>((size_t)ag.yparts * s.nx * s.nz, 0.0));
   return ag;

@@ -109,6 +109,7 @@ struct AdjGeo {
   double* gy_part;       // [yparts][nx][nz] ... of the y-line kernels (y faces and z faces); the
                          // line kernels use part 0, the streaming kernel one part per block of a line
   int32_t yparts;        // blocks per y-line tile of the streaming kernel (adj_yparts)
+  int32_t ncls;          // column classes of the scenario
 };
 
 // Adjoint line modes (mu in, mu out; gradients added to the part buffers):
