@@ -43,7 +43,9 @@ GW_DEV void add_line_part(double acc, double* part, const LineTile& T, int nz, d
 // Without EY, m = mu (the transposed z-step alone).  Gradients go to slot
 // (part, line) of gy_part.
 template <bool EY>
-__global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo ag, double* out, const double* mu,
+// Eight waves per SIMD: the walk is latency-bound, so occupancy buys more than
+// deeper per-wave prefetch (64 registers fit without spills).
+__global__ __launch_bounds__(64 * kStreamWaves, 8) void k_adj_yz(DevView v, AdjGeo ag, double* out, const double* mu,
                                                              const double* u, double c, double a1, double a2,
                                                              double a3) {
   // z coefficients [ncls][64][2] and rock terms [ncls][64] of the scenario's
@@ -58,6 +60,9 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   constexpr int kJs = kStreamWaves * kWalk + 4;
   __shared__ double shHy[kJs], shIy[kJs];
   __shared__ int shCl[kJs];
+  // explicit-y coefficients ea, eb of the tile's coefficient sets (at most
+  // four: the sets of its lanes) for the same elements
+  __shared__ double shEa[4 * kJs], shEb[4 * kJs];
   const int parts = ag.yparts;
   const int64_t tb = blockIdx.x / parts;
   const int p = (int)(blockIdx.x - tb * parts);
@@ -68,10 +73,12 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   const int k = T.kw + lane;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
   const int ncls = ncls_l;
-  // the lane's explicit-y set (zone -> set id -> set offset, three dependent
-  // loads) starts before the z-coefficient staging so the two overlap
-  int64_t so = 0;
-  if (EY) so = v.set_off[v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]]];
+  // the lane's explicit-y set (zone -> set id: its slot among the tile's sets)
+  int eslot = 0;
+  if (EY) {
+    const int sid = v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]];
+    eslot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : (sid == T.set[3] ? 3 : 0));
+  }
   // block's elements j (clamped) from jlo: la - 1 .. lb + 2 of its waves
   const int jlo = max(0, p * kStreamWaves * kWalk - 1);
   for (int q = threadIdx.x; q < kJs; q += 64 * kStreamWaves) {
@@ -80,6 +87,14 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
     shIy[q] = ag.iy[jj > 0 ? jj - 1 : 0];
     shCl[q] = v.cls[S.o_cls + (int64_t)i * S.ny + jj];
   }
+  if (EY)
+    for (int q = threadIdx.x; q < T.nset * kJs; q += 64 * kStreamWaves) {
+      const int s = q / kJs, jj = min(jlo + q - s * kJs, N - 1);
+      const int sq = s == 0 ? T.set[0] : (s == 1 ? T.set[1] : (s == 2 ? T.set[2] : T.set[3]));
+      const double* g = v.coef + v.set_off[sq];
+      shEa[q] = g[SET_EA * N + jj];
+      shEb[q] = g[SET_EB * N + jj];
+    }
   // z coefficients and rock terms of the column classes: all of a wave's loads
   // before its LDS writes (one round trip)
   {
@@ -110,14 +125,6 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
   const double hxi = ag.hx[i];
   const bool kin = k >= 0 && k < nz;
   const bool outl = k >= T.ko0 && k < T.ko1;
-  // explicit-y coefficients of this lane's level along line i: byte offsets
-  // of the lane's set arrays from the coefficient table (uniform base + a
-  // constant 32-bit lane offset per load)
-  unsigned ofa = 0, ofb = 0;
-  if (EY) {
-    ofa = (unsigned)((so + SET_EA * (int64_t)N) * (int64_t)sizeof(double));
-    ofb = (unsigned)((so + SET_EB * (int64_t)N) * (int64_t)sizeof(double));
-  }
   // each wave walks kWalk elements (fewer at the end of the line)
   const int la = min(N, (p * kStreamWaves + w) * kWalk), lb = min(N, la + kWalk);
   // element j of this lane: wave-uniform base of column (i, j) plus the
@@ -154,17 +161,12 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
     // operands in flight in 4-slot register rings (slot of element j: (j - la)
     // % 4, the loop unrolled 4 times so every slot index is a constant and no
     // value moves between registers while its load is in flight): mu three
-    // elements ahead (the explicit step needs mu_{j+1}), u and the
-    // coefficients two ahead
+    // elements ahead (the explicit step needs mu_{j+1}), u two ahead (the
+    // per-element coefficients come from LDS)
     auto jc = [&](int j) { return j < N ? j : N - 1; };
-    double rm[4], ru[4], rea[4], reb[4];
+    double rm[4], ru[4];
     auto load_mu = [&](int slot, int j) { rm[slot & 3] = at(mu, jc(j)); };
-    auto load_rest = [&](int slot, int j) {
-      const int jj = jc(j);
-      ru[slot & 3] = at(u, jj);
-      rea[slot & 3] = EY ? *lane_ptr(v.coef + jj, ofa) : 0.0;
-      reb[slot & 3] = EY ? *lane_ptr(v.coef + jj, ofb) : 0.0;
-    };
+    auto load_rest = [&](int slot, int j) { ru[slot & 3] = at(u, jc(j)); };
 #pragma unroll
     for (int q = 0; q < 3; ++q) load_mu(q, la + q);
 #pragma unroll
@@ -186,7 +188,8 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_adj_yz(DevView v, AdjGeo 
         const double z = u_cur + a3 * Az(n2);
         const double fy = shIy[jj - jlo] * (m_cur - m_prev) * (z - z_prev) * (r_prev + rk);  // used for j > 0 only
         gy += (act && j > 0) ? fy : 0.0;
-        m = m_cur + c * (rea[s & 3] * (m_nx - m_cur) - reb[s & 3] * (m_cur - m_prev));
+        const double ea = shEa[eslot * kJs + jj - jlo], eb = shEb[eslot * kJs + jj - jlo];
+        m = m_cur + c * (ea * (m_nx - m_cur) - eb * (m_cur - m_prev));
         z_prev = z;
         r_prev = rk;
       }
