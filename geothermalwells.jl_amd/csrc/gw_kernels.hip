@@ -187,8 +187,8 @@ __global__ __launch_bounds__(kFinThreads) void k_pipe_final(DevView v, double* j
 // One block per advection group (up to kAdvGroupMax pipe columns of one
 // borehole and one kind, inner or annulus): the departure entries of the
 // thread's levels (t + 256 r) and the columns' offsets are loaded once for the
-// group; each column is staged in LDS (double-buffered, the next column's
-// loads in flight meanwhile) so the update runs in place.
+// group; the columns are staged in LDS two at a time (the next pair's loads in
+// flight meanwhile) so the update runs in place.
 template <class TS, int R>
 __global__ __launch_bounds__(256) void k_pipe_advect(DevView v, TS* __restrict__ phi) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(256) void k_pipe_advect(DevView v, TS* __restrict__
   }
   // Columns go in pairs: both of a pair are staged in the two LDS buffers
   // while the next pair's loads are in flight (a load index past the group
-  // repeats its last column; such values are never written back).
+  // repeats its last column; such values are never used).
   auto load_col = [&](TS (&dst)[R], int c) {
     const TS* pn = colp(c < ncol ? c : ncol - 1);
 #pragma unroll
