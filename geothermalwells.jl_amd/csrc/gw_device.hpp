@@ -33,6 +33,13 @@ GW_DEV void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// A wave-uniform element offset pinned in SGPRs: base + offset + lane byte
+// offset then uses the scalar-base + 32-bit vector-offset addressing form.
+GW_DEV int64_t soff(int64_t x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
+
 // Marks a wave-uniform 64-bit value as such (it then lives in SGPRs).
 GW_DEV int64_t uniform64(int64_t x) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)x);

@@ -218,13 +218,15 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   const int l0 = seg * S;
   const int64_t ustride = uniform64(stride);
   const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
-  const TS* pin = in + useg;
+  // element r of the lane's segment: field base + an element offset kept in
+  // SGPRs (soff; left to itself the compiler forms 64-bit vector addresses,
+  // two to three VALU instructions per access) + the lane's byte offset
   if (l0 + S <= N) {
 #pragma unroll
-    for (int r = 0; r < S; ++r) d[r] = (RT)*lane_ptr(pin + (int64_t)r * ustride, kb);
+    for (int r = 0; r < S; ++r) d[r] = (RT)*lane_ptr(in + soff(useg + (int64_t)r * ustride), kb);
   } else {
 #pragma unroll
-    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? (RT)*lane_ptr(pin + (int64_t)r * ustride, kb) : (RT)0;
+    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? (RT)*lane_ptr(in + soff(useg + (int64_t)r * ustride), kb) : (RT)0;
   }
   __syncthreads();
 
@@ -351,10 +353,9 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   }
 
   if (MT::keep && k >= T.ko0 && k < T.ko1) {
-    TS* pk = static_cast<TS*>(v.aux) + useg;
 #pragma unroll
     for (int r = 0; r < S; ++r)
-      if (l0 + r < N) *lane_ptr(pk + (int64_t)r * ustride, kb) = (TS)d[r];
+      if (l0 + r < N) *lane_ptr(static_cast<TS*>(v.aux) + soff(useg + (int64_t)r * ustride), kb) = (TS)d[r];
   }
 
   if (MT::halo) {
@@ -436,14 +437,13 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   }
 
   if (k >= T.ko0 && k < T.ko1) {
-    TS* pout = out + useg;
     if (l0 + S <= N) {
 #pragma unroll
-      for (int r = 0; r < S; ++r) *lane_ptr(pout + (int64_t)r * ustride, kb) = (TS)d[r];
+      for (int r = 0; r < S; ++r) *lane_ptr(out + soff(useg + (int64_t)r * ustride), kb) = (TS)d[r];
     } else {
 #pragma unroll
       for (int r = 0; r < S; ++r)
-        if (l0 + r < N) *lane_ptr(pout + (int64_t)r * ustride, kb) = (TS)d[r];
+        if (l0 + r < N) *lane_ptr(out + soff(useg + (int64_t)r * ustride), kb) = (TS)d[r];
     }
   }
 }
