@@ -109,9 +109,8 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
   const int64_t ustride = uniform64(stride);
   const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
   {
-    const double* pin = in + useg;
 #pragma unroll
-    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? *lane_ptr(pin + (int64_t)r * ustride, kb) : 0.0;
+    for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? *lane_ptr(in + soff(useg + (int64_t)r * ustride), kb) : 0.0;
   }
   __syncthreads();
 
@@ -134,15 +133,14 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
     __syncthreads();
     const double mu_edge = (l0 + S < N) ? shA[(seg + 1) * LW + kk] : 0.0;
     __syncthreads();
-    const double* px = x + useg;
     // x in chunks of KG elements, the next chunk's loads in flight while the
     // current one computes; the barrier keeps the compiler from hoisting every
     // load of the segment (which would spill the line)
     constexpr int KG = S % 8 == 0 ? 8 : 4;
-    const double xafter = (l0 + S < N) ? *lane_ptr(px + (int64_t)S * ustride, kb) : 0.0;
+    const double xafter = (l0 + S < N) ? *lane_ptr(x + soff(useg + (int64_t)S * ustride), kb) : 0.0;
     double xn[KG];
 #pragma unroll
-    for (int q = 0; q < KG; ++q) xn[q] = (l0 + q < N) ? *lane_ptr(px + (int64_t)q * ustride, kb) : 0.0;
+    for (int q = 0; q < KG; ++q) xn[q] = (l0 + q < N) ? *lane_ptr(x + soff(useg + (int64_t)q * ustride), kb) : 0.0;
     double gsum = 0.0;
 #pragma unroll
     for (int r0 = 0; r0 < S; r0 += KG) {
@@ -152,7 +150,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
       if (r0 + KG < S) {
 #pragma unroll
         for (int q = 0; q < KG; ++q)
-          xn[q] = (l0 + r0 + KG + q < N) ? *lane_ptr(px + (int64_t)(r0 + KG + q) * ustride, kb) : 0.0;
+          xn[q] = (l0 + r0 + KG + q < N) ? *lane_ptr(x + soff(useg + (int64_t)(r0 + KG + q) * ustride), kb) : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < KG; ++q) {
@@ -340,10 +338,9 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
   }
 
   if (k >= T.ko0 && k < T.ko1) {
-    double* pout = out + useg;
 #pragma unroll
     for (int r = 0; r < S; ++r)
-      if (l0 + r < N) *lane_ptr(pout + (int64_t)r * ustride, kb) = d[r];
+      if (l0 + r < N) *lane_ptr(out + soff(useg + (int64_t)r * ustride), kb) = d[r];
   }
 }
 
