@@ -112,6 +112,12 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW] carries / edges
   double* shB = shA + G * LW;                          // [G][LW]
   double* shZ = shB + G * LW;                          // [kZcMax][LW][2]: z coefficients (halo modes)
+  // The explicit step and the z-step run in the state type; in the FP32 64-lane
+  // kernels their coefficients are stored in it (rounded once at staging, not
+  // at every use: the same values).  Elsewhere (FP64, and the FP32 32-lane
+  // kernels, whose register allocation it upsets) they stay FP64.
+  using CT = typename std::conditional<sizeof(TS) == 4 && LW == 64, TS, double>::type;
+  CT* shZt = reinterpret_cast<CT*>(shZ);
   int* shK = reinterpret_cast<int*>(shZ + kZcMax * LW * 2);  // [NM] column class per line element
   double* shC = shA + line_lds_fixed<Cfg, MODE>();     // [nset][MT::NA][NM] coefficient arrays
 
@@ -173,8 +179,13 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
           dst[MT::A_BU * NM + l] = cv[i][2];
         }
         if (MT::expl) {
-          dst[MT::A_EA * NM + l] = cv[i][3];
-          dst[MT::A_EB * NM + l] = cv[i][4];
+          if constexpr (std::is_same<CT, double>::value) {
+            dst[MT::A_EA * NM + l] = cv[i][3];
+            dst[MT::A_EB * NM + l] = cv[i][4];
+          } else {
+            reinterpret_cast<CT*>(dst + MT::A_EA * NM)[l] = (CT)cv[i][3];
+            reinterpret_cast<CT*>(dst + MT::A_EB * NM)[l] = (CT)cv[i][4];
+          }
         }
       }
     }
@@ -195,8 +206,13 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
       for (int i = 0; i < 4; ++i) {
         const int q = q0 + i * Cfg::NW;
         if (q < ncls) {
-          shZ[(q * LW + kk) * 2 + hh] = z0[i];
-          if (H == 1) shZ[(q * LW + kk) * 2 + 1] = z1[i];
+          if constexpr (std::is_same<CT, double>::value) {
+            shZ[(q * LW + kk) * 2 + hh] = z0[i];
+            if (H == 1) shZ[(q * LW + kk) * 2 + 1] = z1[i];
+          } else {
+            shZt[(q * LW + kk) * 2 + hh] = (CT)z0[i];
+            if (H == 1) shZt[(q * LW + kk) * 2 + 1] = (CT)z1[i];
+          }
         }
       }
     }
@@ -234,8 +250,8 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   const double* bd = cs + MT::A_BD * NM;
   const double* fw = cs + MT::A_FW * NM;
   const double* bu = cs + MT::A_BU * NM;
-  const double* ea = cs + MT::A_EA * NM;
-  const double* eb = cs + MT::A_EB * NM;
+  const CT* ea = reinterpret_cast<const CT*>(cs + MT::A_EA * NM);
+  const CT* eb = reinterpret_cast<const CT*>(cs + MT::A_EB * NM);
 
   // Every element loop below runs in chunks of KC elements: the LDS operands
   // of the next chunk are fetched while the current one computes.  The long
@@ -367,8 +383,8 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
 #pragma unroll
     for (int q = 0; q < KZ; ++q) {
       const int cl = shK[l0 + q];
-      ean[q] = (TS)shZ[(cl * LW + kk) * 2];
-      ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
+      ean[q] = (TS)shZt[(cl * LW + kk) * 2];
+      ebn[q] = (TS)shZt[(cl * LW + kk) * 2 + 1];
     }
 #pragma unroll
     for (int r0 = 0; r0 < S; r0 += KZ) {
@@ -379,8 +395,8 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
 #pragma unroll
         for (int q = 0; q < KZ; ++q) {
           const int cl = shK[l0 + r0 + KZ + q];
-          ean[q] = (TS)shZ[(cl * LW + kk) * 2];
-          ebn[q] = (TS)shZ[(cl * LW + kk) * 2 + 1];
+          ean[q] = (TS)shZt[(cl * LW + kk) * 2];
+          ebn[q] = (TS)shZt[(cl * LW + kk) * 2 + 1];
         }
       }
       // the chunk's KZ elements advance through the three stages together
