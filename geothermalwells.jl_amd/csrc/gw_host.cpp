@@ -1077,10 +1077,11 @@ int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
                           "line tune");
       };
       if ((rc = run())) break;  // first launch: code object load
-      // Fastest of three separately timed launches: a single slow launch
-      // (a neighbour's clock or cache state) must not decide the variant.
+      // Fastest of five separately timed launches: a single slow launch
+      // (clock or cache state) must not decide the variant, and small fields
+      // launch in tens of microseconds, close to the timing noise.
       float ms = 0.f;
-      for (int r = 0; r < 3 && !rc; ++r) {
+      for (int r = 0; r < 5 && !rc; ++r) {
         if ((rc = hip_status(hipEventRecord(e0, ctx.stream), "event"))) break;
         if ((rc = run())) break;
         if ((rc = hip_status(hipEventRecord(e1, ctx.stream), "event"))) break;
