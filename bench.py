@@ -340,8 +340,9 @@ def main():
                     help="first run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the hot path is single-device in this build (see DESIGN.md on "
-              "sharding); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the step pipeline runs on a single device (a field split over "
+              "GPUs would shard as z-slabs with a 3-level halo exchange per step, not implemented); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
