@@ -40,18 +40,19 @@ def oracle_for(w):
 
 
 def small_case(nbh=1, layered=False, inlet="constant", h=60.0, nz_dz=(70.0, 2.0), n_fine=0.01, zins=0.0,
-               depths=(20.0, 45.0, 65.0)):
+               depths=(20.0, 45.0, 65.0), xs=None, hs=None):
     """A few-thousand-cell problem that still exercises every material class,
     the turnaround and the inlet.  With depths (20, 57, 65) a layer boundary
     lies two levels above the 60 m bottom: 7 consecutive levels then meet 4
     coefficient zones (two layers with the pipe, the casing's bottom level,
-    rock alone)."""
+    rock alone).  xs, hs: borehole centres and depths (default: h for all)."""
     g = gw()
-    if nbh == 1:
-        xs = [(0.0, 0.0)]
-    else:
-        xs = [(-3.0, 0.0), (3.0, 0.5)][:nbh]
-    bhs = tuple(g.Borehole(x, y, h, 0.038, 0.0065, 0.0889, 0.01, 0.108, 0.01 * 998.2, zins) for x, y in xs)
+    if xs is None:
+        xs = [(0.0, 0.0)] if nbh == 1 else [(-3.0, 0.0), (3.0, 0.5)][:nbh]
+    if hs is None:
+        hs = [h] * len(xs)
+    bhs = tuple(g.Borehole(x, y, hb, 0.038, 0.0065, 0.0889, 0.01, 0.108, 0.01 * 998.2, zins)
+                for (x, y), hb in zip(xs, hs))
     if layered:
         mat = g.StratifiedMaterialProperties((1.8, 2.6, 3.5), (2.45e6, 2.94e6, 1.96e6), tuple(depths), 0.6,
                                              4.18e6, 44.5, 3.73e6, 0.4, 1.96e6, 1.5, 1.76e6)

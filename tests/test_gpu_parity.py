@@ -49,6 +49,13 @@ CASES = {
     "two_bh_hx": dict(nbh=2, inlet="hx"),
     "two_bh_perwell_ins": dict(nbh=2, inlet="perwell", zins=20.0),
     "layer_above_bottom": dict(layered=True, depths=(20.0, 57.0, 65.0)),
+    # the shorter borehole's levels between its bottom (+ casing) and the
+    # deeper one's are pipe columns the advection must leave unchanged
+    "two_bh_depths": dict(nbh=2, inlet="perwell", hs=(60.0, 41.0)),
+    # 0.2 m apart: the second borehole's backfill disc reaches the first
+    # one's outer steel ring, and (no early exit) its backfill conductivity
+    # wins there while the heat capacity stays steel
+    "two_bh_touching": dict(nbh=2, inlet="hx", xs=[(0.0, 0.0), (0.2, 0.0)], hs=(60.0, 50.0)),
 }
 
 

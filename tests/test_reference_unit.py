@@ -164,3 +164,28 @@ def test_point_material_classification(gw, O):
     # every material value occurs
     assert set(np.unique(km)) == {2.5, 0.6, 0.4, 44.5, 1.5}
     assert set(np.unique(rcm)) == {2.3e6, 4.18e6, 1.96e6, 3.73e6}
+
+
+def test_later_borehole_overrides_conductivity(gw, O):
+    """src/physics.jl:204-233 visits every borehole in order with no early
+    exit, so a later borehole's backfill disc overrides an earlier one's steel
+    ring for k; the heat capacity (src/physics.jl:248-274) only looks within
+    r_outer + t_outer, so there the steel value stays.  Known answers at
+    x = 0.095 m: 0.095 m from the first borehole (steel ring 0.0889-0.0989 m)
+    and 0.105 m from the second (backfill up to 0.108 m)."""
+    from geothermalwells_jl_amd import physics as P
+    m = gw.HomogenousMaterialProperties(2.5, 2.3e6, 0.6, 4.18e6, 44.5, 3.73e6, 0.4, 1.96e6, 1.5, 1.76e6)
+    a = gw.Borehole(0.0, 0.0, 60.0, 0.038, 0.0065, 0.0889, 0.01, 0.108, 10.0, 0.0)
+    b = gw.Borehole(0.2, 0.0, 50.0, 0.038, 0.0065, 0.0889, 0.01, 0.108, 10.0, 0.0)
+    k_steel, k_backfill, rc_steel = 44.5, 1.5, 3.73e6
+    # z = 20: within both boreholes' depth; z = 56: below b's casing plug (53 m), above a's bottom
+    expect = {(a, b): {20.0: k_backfill, 56.0: k_steel}, (b, a): {20.0: k_steel, 56.0: k_steel}}
+    for bhs, by_z in expect.items():
+        w = dict(gridx=np.array([-0.3, 0.095, 0.4]), gridy=np.array([-0.2, 0.0, 0.2]),
+                 gridz=np.array([0.0, 20.0, 56.0, 70.0]), materials=m, boreholes=bhs,
+                 inlet_model=gw.ConstantInlet(20.0))
+        km, rcm = H.oracle_for(w).material_maps()
+        for kz, z in ((1, 20.0), (2, 56.0)):
+            assert P.get_thermal_conductivity(0.095, 0.0, z, bhs, m) == by_z[z]
+            assert P.get_volumetric_heat_capacity(0.095, 0.0, z, bhs, m) == rc_steel
+            assert km[1, 1, kz] == by_z[z] and rcm[1, 1, kz] == rc_steel
