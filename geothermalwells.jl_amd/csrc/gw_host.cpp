@@ -568,11 +568,13 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   std::vector<Dep> dep;
   std::vector<BhDev> hb(ctx.nbh_total);
   std::vector<RedChunk> chunks, groups;
-  // columns per advection block: about 2048 blocks per launch, 1 to 8 columns
-  // each (a block walks its columns one after another)
+  // columns per advection block: at most about 2048 blocks per launch (all of
+  // them resident at once, 8 per CU, instead of a second partial round), 1 to
+  // 8 columns each (a block walks its columns one after another, the next
+  // column's loads in flight)
   int64_t npipe = 0;
   for (int b = 0; b < ctx.nbh_total; ++b) npipe += ctx.bh_n_in[b] + ctx.bh_n_an[b];
-  const int kAdvGroup = (int)std::min<int64_t>(kAdvGroupMax, std::max<int64_t>(1, npipe / 2048));
+  const int kAdvGroup = (int)std::min<int64_t>(kAdvGroupMax, std::max<int64_t>(1, (npipe + 2047) / 2048));
   int gb = 0, mean_off = 0, poff = 0;
   for (size_t si = 0; si < ctx.sc.size(); ++si) {
     const Scen& s = ctx.sc[si];

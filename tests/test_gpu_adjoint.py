@@ -50,6 +50,18 @@ def test_gradient_matches_dual_oracle(layered, inlet, nbh, seg, dt, depths):
     assert np.abs(got_dT - ref_dT).max() <= 1e-7 * np.abs(ref_dT).max(), (got_dT, ref_dT)
 
 
+def test_gradient_with_multi_column_advection_groups():
+    """2581 pipe columns (505 inner, 2076 annulus): the advection and its
+    transpose run blocks of 2 columns, the inner pipe's last column alone."""
+    g, cache, orc, u0, d = _setup(H.small_case(n_fine=0.003, nz_dz=(70.0, 4.0)))
+    nsteps, dt = 3, 60.0
+    J, dk, dT = g.outlet_gradient(cache, d, 0.0, dt, nsteps, checkpoint_every=2)
+    Jo, dJo = orc.objective_dual(u0, 0.0, dt, nsteps, cache.rock2_c3)
+    assert abs(J - Jo) <= 1e-9 * abs(Jo)
+    assert abs(dk[0] - dJo[0]) <= 1e-7 * abs(dJo[0]), (dk, dJo)
+    assert abs(dT.sum() - dJo[1]) <= 1e-7 * abs(dJo[1]), (dT, dJo)
+
+
 def test_state_gradient_is_exact_for_the_affine_map():
     """dJ/du0 against (J(u0 + e d) - J(u0 - e d)) / 2e: J is affine in u0, so
     the quotient is exact up to rounding."""

@@ -56,6 +56,10 @@ CASES = {
     # one's outer steel ring, and (no early exit) its backfill conductivity
     # wins there while the heat capacity stays steel
     "two_bh_touching": dict(nbh=2, inlet="hx", xs=[(0.0, 0.0), (0.2, 0.0)], hs=(60.0, 50.0)),
+    # 5781 pipe columns: advection blocks of 3 columns (a pair, then one)
+    "fine_pipe_groups_of_3": dict(layered=True, n_fine=0.002, nz_dz=(70.0, 4.0)),
+    # 2581 pipe columns: blocks of 2, the inner pipe's odd last column alone
+    "fine_pipe_groups_of_2": dict(inlet="hx", n_fine=0.003, nz_dz=(70.0, 4.0)),
 }
 
 
