@@ -296,6 +296,7 @@ struct gw_ctx {
       }
     }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
+    for (const Scen& s : sc) v.zc_cls = std::max(v.zc_cls, std::min(s.ncls, kZcMax));
     if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial;
              v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
@@ -950,7 +951,7 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       ctx->d_htiles[f] = ctx->up(ht[f]);
     }
     ctx->fused_ok = true;
-    for (auto& s : ctx->sc) ctx->fused_ok = ctx->fused_ok && s.ncls <= 16;
+    for (auto& s : ctx->sc) ctx->fused_ok = ctx->fused_ok && s.ncls <= kZcMax;
     ctx->d_cls = ctx->up(cls);
     ctx->d_zone = ctx->up(zone);
     ctx->d_zcoef = ctx->up(zcoef);

@@ -22,6 +22,7 @@ inline int line_class(int n) {
 // borehole, so the summation order does not depend on the batch.
 constexpr int kRedChunk = 16;
 constexpr int kAdvGroupMax = 8;  // most pipe columns of an advection group
+constexpr int kZcMax = 16;       // most column classes a scenario may have for the fused z-step kernels
 
 struct DevView {
   const ScenDev* sc;
@@ -63,6 +64,7 @@ struct DevView {
   void* aux;                    // LINE_IMPLICIT_EXPLICIT_KEEP: receives the implicit result
   int32_t lds_floor;            // bytes: each line tile requests at least this much LDS
   int32_t halo_whole;           // 1: the fused z-step modes launch each class as one range
+  int32_t zc_cls;               // column classes whose z coefficients a halo tile stages (most of any scenario)
 };
 
 // The k-window argument of launch_line is 64 or 32, optionally with

@@ -55,7 +55,6 @@ struct LineCfg {
 
 constexpr int kSlots = 3;      // coefficient sets per plain window (LDS slots)
 constexpr int kSlotsHalo = 4;  // most sets per z-halo window
-constexpr int kZcMax = 16;     // column classes staged for the fused z-step
 
 template <int MODE> struct ModeTraits {
   static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY;
@@ -69,17 +68,26 @@ template <int MODE> struct ModeTraits {
   static constexpr int A_BD = 0, A_FW = 1, A_BU = 2, A_EA = impl ? 3 : 0, A_EB = A_EA + 1;
 };
 
-// LDS of a tile, in doubles: carries and edges, the z coefficients and column
-// classes of the halo modes, then `slots` coefficient-set slots (last, so the
-// slot count only changes the launch's LDS size).
-template <class Cfg, int MODE>
-constexpr size_t line_lds_fixed() {
+// Type of the explicit-step and z-step coefficients a tile keeps in LDS: the
+// state type in the FP32 64-lane kernels (rounded once at staging, not at
+// every use: the same values), FP64 elsewhere (the FP32 32-lane kernels'
+// register allocation does worse with it).
+template <class TS, int LW>
+using LineCT = typename std::conditional<sizeof(TS) == 4 && LW == 64, TS, double>::type;
+
+// LDS of a tile, in doubles: carries and edges, the z coefficients of the
+// zcls column classes and the column class per element (halo modes), then
+// `slots` coefficient-set slots (last, so the slot count only changes the
+// launch's LDS size).
+template <class Cfg, class TS, int MODE>
+__host__ __device__ inline size_t line_lds_fixed(int zcls) {
   using MT = ModeTraits<MODE>;
-  return 2 * Cfg::G * Cfg::LW + (MT::halo ? kZcMax * Cfg::LW * 2 + (Cfg::NMAX + 1) / 2 : 0);
+  constexpr size_t zc_per_cls = Cfg::LW * 2 * sizeof(LineCT<TS, Cfg::LW>) / sizeof(double);
+  return 2 * Cfg::G * Cfg::LW + (MT::halo ? (size_t)zcls * zc_per_cls + (Cfg::NMAX + 1) / 2 : 0);
 }
-template <class Cfg, int MODE>
-constexpr size_t line_lds_doubles(int slots) {
-  return line_lds_fixed<Cfg, MODE>() + (size_t)slots * ModeTraits<MODE>::NA * Cfg::NMAX;
+template <class Cfg, class TS, int MODE>
+size_t line_lds_doubles(int slots, int zcls) {
+  return line_lds_fixed<Cfg, TS, MODE>(zcls) + (size_t)slots * ModeTraits<MODE>::NA * Cfg::NMAX;
 }
 
 GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
@@ -111,15 +119,14 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW] carries / edges
   double* shB = shA + G * LW;                          // [G][LW]
-  double* shZ = shB + G * LW;                          // [kZcMax][LW][2]: z coefficients (halo modes)
-  // The explicit step and the z-step run in the state type; in the FP32 64-lane
-  // kernels their coefficients are stored in it (rounded once at staging, not
-  // at every use: the same values).  Elsewhere (FP64, and the FP32 32-lane
-  // kernels, whose register allocation it upsets) they stay FP64.
-  using CT = typename std::conditional<sizeof(TS) == 4 && LW == 64, TS, double>::type;
+  double* shZ = shB + G * LW;                          // [zcls][LW][2]: z coefficients (halo modes)
+  // The explicit step and the z-step run in the state type, their LDS
+  // coefficients are of type CT (LineCT).
+  using CT = LineCT<TS, LW>;
   CT* shZt = reinterpret_cast<CT*>(shZ);
-  int* shK = reinterpret_cast<int*>(shZ + kZcMax * LW * 2);  // [NM] column class per line element
-  double* shC = shA + line_lds_fixed<Cfg, MODE>();     // [nset][MT::NA][NM] coefficient arrays
+  const int zcls = v.zc_cls;
+  int* shK = reinterpret_cast<int*>(shZt + zcls * LW * 2);  // [NM] column class per line element
+  double* shC = shA + line_lds_fixed<Cfg, TS, MODE>(zcls);  // [nset][MT::NA][NM] coefficient arrays
 
   constexpr int FAM = LW == 32 ? 1 : 0;
   const LineTile* tiles = MT::halo ? v.htiles[FAM] : (dir == 0 ? v.xtiles[FAM] : v.ytiles[FAM]);
@@ -468,7 +475,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
 template <class Cfg, class TS, int MODE>
 static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
                         double a3, int64_t tile0, int64_t ntiles, int slots, hipStream_t st) {
-  const size_t sh = std::max(line_lds_doubles<Cfg, MODE>(slots) * sizeof(double), (size_t)v.lds_floor);
+  const size_t sh = std::max(line_lds_doubles<Cfg, TS, MODE>(slots, v.zc_cls) * sizeof(double), (size_t)v.lds_floor);
   hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
                      c, a1, a2, a3, tile0, ntiles);
 }
