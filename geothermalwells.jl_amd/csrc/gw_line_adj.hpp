@@ -23,12 +23,13 @@ template <int MODE> struct AdjTraits {
 };
 
 template <class Cfg, int MODE>
-constexpr size_t line_adj_lds_doubles() {
+size_t line_adj_lds_doubles(int zcls) {
   using AT = AdjTraits<MODE>;
-  // carries/edges, coefficient sets, z coefficients (halo), rock flags,
-  // inverse face spacing and h along the line, class per element
-  return 2 * Cfg::G * Cfg::LW + kSlots * AT::NA * Cfg::NMAX + (AT::halo ? kZcMax * Cfg::LW * 2 : 0) +
-         kZcMax * Cfg::LW + 2 * Cfg::NMAX + (Cfg::NMAX + 1) / 2;
+  // carries/edges, coefficient sets, z coefficients (halo) and rock flags of
+  // the zcls column classes, inverse face spacing and h along the line, class
+  // per element
+  return 2 * Cfg::G * Cfg::LW + kSlots * AT::NA * Cfg::NMAX + (size_t)zcls * Cfg::LW * (AT::halo ? 3 : 1) +
+         2 * Cfg::NMAX + (Cfg::NMAX + 1) / 2;
 }
 
 template <class Cfg, int MODE>
@@ -43,9 +44,10 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
   double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW]
   double* shB = shA + G * LW;                          // [G][LW]
   double* shC = shB + G * LW;                          // [kSlots][NA][NM]
-  double* shZ = shC + kSlots * AT::NA * NM;            // [kZcMax][LW][2] (halo modes)
-  double* shR = shZ + (AT::halo ? kZcMax * LW * 2 : 0);  // [kZcMax][LW] rock flags
-  double* shI = shR + kZcMax * LW;                     // [NM] 1 / face spacing along the line
+  const int zcls = v.zc_cls;                           // column classes staged
+  double* shZ = shC + kSlots * AT::NA * NM;            // [zcls][LW][2] (halo modes)
+  double* shR = shZ + (AT::halo ? zcls * LW * 2 : 0);  // [zcls][LW] rock flags
+  double* shI = shR + zcls * LW;                       // [NM] 1 / face spacing along the line
   double* shH = shI + NM;                              // [NM] control length along the line
   int* shK = reinterpret_cast<int*>(shH + NM);         // [NM] column class per element
 
@@ -348,7 +350,7 @@ template <class Cfg, int MODE>
 static void line_adj_launch(const DevView& v, const AdjGeo& ag, int dir, double* out, const double* in,
                             const double* x1, const double* x2, double c, double a1, double a2, double a3,
                             int64_t tile0, int64_t ntiles, hipStream_t st) {
-  const size_t sh = line_adj_lds_doubles<Cfg, MODE>() * sizeof(double);
+  const size_t sh = line_adj_lds_doubles<Cfg, MODE>(v.zc_cls) * sizeof(double);
   hipLaunchKernelGGL((k_line_adj<Cfg, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, ag, dir, out,
                      in, x1, x2, c, a1, a2, a3, tile0, ntiles);
 }
