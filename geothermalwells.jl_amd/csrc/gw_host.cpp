@@ -1524,23 +1524,38 @@ int run_fused(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, int 
   return fused_rest(ctx, tab, (n & 1) ? B : phi, phi, t + n * dt, dt, true, jacc);
 }
 
-// One step from u with the fused z-step + explicit y kernel, the fused x
-// kernel keeping e = (I - cA_x)^-1 b, and the implicit y; u_next receives the
-// step's result.
-int adj_recompute(gw_ctx& ctx, const CTab* tab, const double* u, double* e, double* u_next, double* wb,
-                  double t_end, double dt) {
+// The n steps of a segment from u_0 = U[0]: the z-step + explicit y of the
+// first step, then per step m the two advection sub-steps around the fused x
+// kernel keeping e_m = (I - cA_x)^-1 b in E[m], and the implicit y keeping
+// u_{m+1} in U[m+1], fused with the next step's z-step + explicit y except
+// after the last step.  wa, wb: work fields (the fused y kernel writes out of
+// place).
+int adj_recompute(gw_ctx& ctx, const CTab* tab, double* const* U, double* const* E, int n, double* wa, double* wb,
+                  double t_first_end, double dt) {
   const double c = dt / 2;
   const DevView v = ctx.view(tab);
-  int rc = hip_status(launch_line(v, GW_F64, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), wb, u, c, dt, ctx.c3, ctx.stream),
+  int rc = hip_status(launch_line(v, GW_F64, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), wa, U[0], c, dt, ctx.c3, ctx.stream),
                       "z-step + explicit y");
-  if (!rc) rc = sub_step_advect(ctx, tab, wb, t_end);
-  DevView vk = v;
-  vk.aux = e;
-  if (!rc) rc = hip_status(launch_line(vk, GW_F64, LINE_IMPLICIT_EXPLICIT_KEEP, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0),
-                                       wb, wb, c, 0.0, 0.0, ctx.stream), "implicit + explicit x");
-  if (!rc) rc = sub_step_advect(ctx, tab, wb, t_end + c);
-  if (!rc) rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), u_next, wb, c, 0.0,
-                                       0.0, ctx.stream), "implicit y");
+  for (int m = 0; m < n && !rc; ++m) {
+    const double t_end = t_first_end + (double)m * dt;
+    rc = sub_step_advect(ctx, tab, wa, t_end);
+    DevView vk = v;
+    vk.aux = E[m];
+    if (!rc) rc = hip_status(launch_line(vk, GW_F64, LINE_IMPLICIT_EXPLICIT_KEEP, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0),
+                                         wa, wa, c, 0.0, 0.0, ctx.stream), "implicit + explicit x");
+    if (!rc) rc = sub_step_advect(ctx, tab, wa, t_end + c);
+    if (rc) break;
+    if (m + 1 < n) {
+      DevView vy = v;
+      vy.aux = U[m + 1];
+      rc = hip_status(launch_line(vy, GW_F64, LINE_YZY_KEEP, 1, ctx.lw(LINE_YZY, 1), wb, wa, c, dt, ctx.c3,
+                                  ctx.stream), "implicit y + z-step + explicit y");
+      std::swap(wa, wb);
+    } else {
+      rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), U[m + 1], wa, c, 0.0, 0.0,
+                                  ctx.stream), "implicit y");
+    }
+  }
   return rc;
 }
 
@@ -1670,7 +1685,7 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     DevBufs db;
     const AdjGeo ag = make_adj_geo(C, db);
     const AdjPipe ap = build_adj_pipe(C, *tab, db);
-    // storage: checkpoints, one segment of u / e, two adjoint fields, a work field
+    // storage: checkpoints, one segment of u / e, two adjoint fields, two work fields
     const int nseg = (nsteps + seg - 1) / seg;
     const int sl = std::min(seg, nsteps);
     std::vector<double*> ck(nseg), U(sl + 1), E(sl);
@@ -1680,6 +1695,7 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     double* MU0 = db.alloc<double>(cells);
     double* MU1 = db.alloc<double>(cells);
     double* WB = db.alloc<double>(cells);
+    double* WB2 = db.alloc<double>(cells);
     double* jacc = db.up(std::vector<double>(nbh, 0.0));
     // forward with checkpoints
     if ((rc = hip_status(hipMemcpyAsync(U[0], phi0, bytes, hipMemcpyDeviceToDevice, C.stream), "copy"))) return rc;
@@ -1724,8 +1740,7 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
       const int base = sg * seg;
       if ((rc = hip_status(hipMemcpyAsync(U[0], ck[sg], bytes, hipMemcpyDeviceToDevice, C.stream), "copy")))
         return rc;
-      for (int m = 0; m < n; ++m)
-        if ((rc = adj_recompute(C, tab, U[m], E[m], U[m + 1], WB, t0 + (double)(base + m + 1) * dt, dt))) return rc;
+      if ((rc = adj_recompute(C, tab, U.data(), E.data(), n, WB, WB2, t0 + (double)(base + 1) * dt, dt))) return rc;
       for (int m = n - 1; m >= 0; --m) {
         if ((rc = adj_line(ADJ_Y, Bf, A, U[m + 1]))) return rc;
         if ((rc = hip_status(launch_adj_pipe(v, ag, ap, Bf, seed, C.stream), "adjoint advection"))) return rc;

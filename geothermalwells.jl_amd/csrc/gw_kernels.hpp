@@ -79,16 +79,18 @@ constexpr int kLwHaloWhole = 2;
 constexpr int kLwFlags = kLwTwoTiles | kLwHaloWhole;
 constexpr int kLdsTwoTiles = 80 * 1024;
 
-// LINE_IMPLICIT_EXPLICIT_KEEP is LINE_IMPLICIT_EXPLICIT that also writes the
-// implicit result to DevView::aux (FP64 only; the adjoint's recompute keeps
-// it).  It shares the k-window width chosen for LINE_IMPLICIT_EXPLICIT.
+// LINE_IMPLICIT_EXPLICIT_KEEP / LINE_YZY_KEEP are LINE_IMPLICIT_EXPLICIT /
+// LINE_YZY that also write the implicit result to DevView::aux (FP64 only;
+// the adjoint's recompute keeps it).  They share the k-window width chosen
+// for LINE_IMPLICIT_EXPLICIT / LINE_YZY.
 enum LineMode {
   LINE_EXPLICIT = 0,
   LINE_IMPLICIT = 1,
   LINE_IMPLICIT_EXPLICIT = 2,
   LINE_ZY = 3,
   LINE_YZY = 4,
-  LINE_IMPLICIT_EXPLICIT_KEEP = 5
+  LINE_IMPLICIT_EXPLICIT_KEEP = 5,
+  LINE_YZY_KEEP = 6
 };
 
 // ---------------------------------------------------------------------------

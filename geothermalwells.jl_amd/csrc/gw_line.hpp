@@ -57,11 +57,11 @@ constexpr int kSlots = 3;      // coefficient sets per plain window (LDS slots)
 constexpr int kSlotsHalo = 4;  // most sets per z-halo window
 
 template <int MODE> struct ModeTraits {
-  static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY;
+  static constexpr bool halo = MODE == LINE_ZY || MODE == LINE_YZY || MODE == LINE_YZY_KEEP;
   static constexpr bool impl = MODE == LINE_IMPLICIT || MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_YZY ||
-                               MODE == LINE_IMPLICIT_EXPLICIT_KEEP;
+                               MODE == LINE_IMPLICIT_EXPLICIT_KEEP || MODE == LINE_YZY_KEEP;
   static constexpr bool expl = MODE != LINE_IMPLICIT;
-  static constexpr bool keep = MODE == LINE_IMPLICIT_EXPLICIT_KEEP;
+  static constexpr bool keep = MODE == LINE_IMPLICIT_EXPLICIT_KEEP || MODE == LINE_YZY_KEEP;
   // coefficient arrays staged per set slot: bd, fw, bu (implicit) then ea, eb
   // (explicit), each NMAX long
   static constexpr int NA = (impl ? 3 : 0) + (expl ? 2 : 0);
@@ -501,6 +501,12 @@ static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const T
         break;
       }
       return hipErrorInvalidValue;
+    case LINE_YZY_KEEP:
+      if constexpr (std::is_same<TS, double>::value) {
+        line_launch<Cfg, TS, LINE_YZY_KEEP>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st);
+        break;
+      }
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -560,7 +566,7 @@ hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const vo
   using F = LineFamily<LW>;
   TS* out = static_cast<TS*>(outp);
   const TS* in = static_cast<const TS*>(inp);
-  const bool halo = mode == LINE_ZY || mode == LINE_YZY;
+  const bool halo = mode == LINE_ZY || mode == LINE_YZY || mode == LINE_YZY_KEEP;
   const int f = LW == 32 ? 1 : 0;
   const int64_t* cls = halo ? v.hcls[f] : (dir == 0 ? v.xcls[f] : v.ycls[f]);
   // halo tiles: those with one coefficient set first, launched with one LDS
