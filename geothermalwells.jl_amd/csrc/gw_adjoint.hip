@@ -238,8 +238,9 @@ GW_DEV double ordered_sum_adj(int n, F ld) {
 }
 
 // Pass 1: per reduction chunk, the column sums of lambda = W mu at the levels
-// the transposed turnaround (inner chunks) or the inlet gradient (annulus
-// chunks) needs.
+// the transposed turnaround (inner chunks: kept per level for pass 2) or the
+// inlet gradient (annulus chunks: summed over the inlet levels, added to the
+// chunk's own accumulator, so the order is fixed) needs.
 __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPipe ap, const double* mu) {
   const RedChunk R = ap.chunk[blockIdx.x];
   const BhDev B = v.bh[R.bh];
@@ -248,55 +249,24 @@ __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPi
   const int k0 = R.part == 0 ? A.t0 : A.i0;
   const int nl = R.part == 0 ? A.nt : A.ni;
   __shared__ AdjCol scol[kRedChunk];  // the chunk's column entries, loaded once
+  __shared__ double red[256];
   if ((int)threadIdx.x < R.ncol) scol[threadIdx.x] = ap.col[R.col0 + threadIdx.x];
   __syncthreads();
+  double g = 0.0;
   for (int t = threadIdx.x; t < nl; t += blockDim.x) {
     const int lev = k0 + t;
     const double hz = ag.hz[lev];
-    ap.partial[R.poff + t] = ordered_sum_adj<kRedChunk>(R.ncol, [&](int c) {
+    const double x = ordered_sum_adj<kRedChunk>(R.ncol, [&](int c) {
       const AdjCol ac = scol[c];
       return ac.hxy * hz * ag.rho[(int64_t)ac.cls * S.nz + lev] * mu[ac.off + lev];
     });
+    if (R.part == 0) ap.partial[R.poff + t] = x;
+    else if (v.dep[B.dep_off + 2 * lev + 1].type == DEP_INLET) g += x;
   }
-}
-
-// Pass 2: per borehole.  Lam_k = sum over the inner chunks (turnaround levels);
-// s_j = (1/n_an) sum over the turnaround entries of level j of w * Lam_k (the
-// adjoint of the annulus level means); the inlet gradient adds the annulus
-// inlet levels' sums.  Level sums in a fixed order (level_sum).
-__global__ __launch_bounds__(kFinThreads) void k_adj_final(DevView v, AdjPipe ap) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  double* lam = reinterpret_cast<double*>(smem_raw);  // [nt]
-  __shared__ double red[kFinThreads];
-  const int b = blockIdx.x;
-  const BhDev B = v.bh[b];
-  const AdjBh A = ap.bh[b];
-  const int t = threadIdx.x, l = t % kFinLP, s = t / kFinLP;
-  // a borehole's chunk partials are contiguous: chunk c of a part at base + c * levels
-  const double* pin = ap.partial + (A.nred_in > 0 ? ap.chunk[A.red0].poff : 0);
-  const double* pan = ap.partial + (A.nred_an > 0 ? ap.chunk[A.red0 + A.nred_in].poff : 0);
-  for (int l0 = 0; l0 < A.nt; l0 += kFinLP) {
-    const int q = l0 + l;
-    const bool turn = q < A.nt && v.dep[B.dep_off + 2 * (A.t0 + q)].type == DEP_TURN;
-    const double x = level_sum(turn ? A.nred_in : 0, [&](int c) { return pin[(int64_t)c * A.nt + q]; }, red);
-    if (s == 0 && q < A.nt) lam[q] = x;
+  if (R.part == 1) {
+    g = block_sum<256>(g, red);
+    if (threadIdx.x == 0) ap.gacc[blockIdx.x] += g;
   }
-  __syncthreads();
-  for (int q = t; q < B.nlev; q += kFinThreads) {
-    double sm = 0.0;
-    for (int e = ap.toff[A.turn_off + q]; e < ap.toff[A.turn_off + q + 1]; ++e)
-      sm += ap.tent[e].w * lam[ap.tent[e].k - A.t0];
-    ap.sbuf[A.soff + q] = sm / (double)B.n_an;
-  }
-  double g = 0.0;
-  for (int l0 = 0; l0 < A.ni; l0 += kFinLP) {
-    const int q = l0 + l;
-    const bool inl = q < A.ni && v.dep[B.dep_off + 2 * (A.i0 + q) + 1].type == DEP_INLET;
-    const double x = level_sum(inl ? A.nred_an : 0, [&](int c) { return pan[(int64_t)c * A.ni + q]; }, red);
-    if (s == 0) g += x;
-  }
-  g = block_sum<kFinThreads>(g, red);
-  if (t == 0) ap.gtin[b] += g;
 }
 
 // Pass 3: per advection group (up to 8 columns of one borehole and kind), for
@@ -333,7 +303,34 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
   // the loop would wait for every access in flight)
   __shared__ AdjCol scol[kAdvGroupMax];
   if (t < Gp.ncol) scol[t] = ap.col[Gp.col0 + t];
+  // annulus groups: the adjoint of the turnaround means at the thread's
+  // levels, s_k = (1/n_an) sum over the turnaround entries of level k of
+  // w Lam_k', with Lam_k' the inner chunks' partials of turnaround level k'
+  // added in chunk order (every block of the borehole gets the same values)
+  double sk[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) sk[r] = 0.0;
+  const bool turn = Gp.part == 1 && A.nt > 0 && A.nred_in > 0;
+  if (turn) {
+    double* lam = lout + v.countz_max;  // [nt]
+    const double* pin = ap.partial + ap.chunk[A.red0].poff;
+    for (int q = t; q < A.nt; q += 256)
+      lam[q] = ordered_sum_adj<8>(A.nred_in, [&](int c) { return pin[(int64_t)c * A.nt + q]; });
+  }
   __syncthreads();
+  if (turn) {
+    const double* lam = lout + v.countz_max;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = t + 256 * r - B.klo;
+      if (t + 256 * r < nlev && q >= 0 && q < B.nlev) {
+        double sm = 0.0;
+        for (int e = ap.toff[A.turn_off + q]; e < ap.toff[A.turn_off + q + 1]; ++e)
+          sm += ap.tent[e].w * lam[ap.tent[e].k - A.t0];
+        sk[r] = sm / (double)B.n_an;
+      }
+    }
+  }
   for (int c = 0; c < Gp.ncol; ++c) {
     const AdjCol ac = scol[c];
     double* p = mu + ac.off;
@@ -352,7 +349,7 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
       if (k < nlev) {
         double lin = g0[r].w * lout[g0[r].k] + g1[r].w * lout[g1[r].k];
         for (int e = e_beg[r] + 2; e < e_end[r]; ++e) lin += ap.ent[e].w * lout[ap.ent[e].k];
-        if (Gp.part == 1 && k >= B.klo && k < B.klo + B.nlev) lin += ap.sbuf[A.soff + k - B.klo];
+        lin += sk[r];
         if (Gp.part == 0 && k < B.n_outlev) lin += sd;
         p[k] = lin / wk[r];
       }
@@ -401,12 +398,8 @@ hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_adj_final, dim3((unsigned)v.nbh), dim3(kFinThreads), (size_t)v.countz_max * sizeof(double), st,
-                     v, ap);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   if (v.nagroup == 0) return hipSuccess;
-  const size_t sh = (size_t)v.countz_max * sizeof(double);
+  const size_t sh = 2 * (size_t)v.countz_max * sizeof(double);  // lout, lam
   const int R = (v.countz_max + 255) / 256;
   const dim3 grid((unsigned)v.nagroup), block(256);
   if (R <= 1) hipLaunchKernelGGL((k_adj_apply<1>), grid, block, sh, st, v, ag, ap, mu, seed);

@@ -1417,7 +1417,7 @@ AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
   std::vector<int32_t> off, toff;
   std::vector<AdjEnt> ent, tent;
   std::vector<RedChunk> ch;
-  int poff = 0, soff = 0;
+  int poff = 0;
   constexpr int kChunk = kRedChunk;
   for (int b = 0; b < ctx.nbh_total; ++b) {
     const BhDev& B = tab.h_bh[b];
@@ -1466,8 +1466,6 @@ AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
       tent.insert(tent.end(), tl[q].begin(), tl[q].end());
     }
     toff.push_back((int32_t)tent.size());
-    A.soff = soff;
-    soff += std::max(B.nlev, 0);
     // reduction chunks: inner columns over the turnaround levels, annulus
     // columns over the inlet levels
     A.red0 = (int32_t)ch.size();
@@ -1506,8 +1504,7 @@ AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
   ap.chunk = db.up(ch);
   ap.nchunk = (int32_t)ch.size();
   ap.partial = db.alloc<double>((size_t)poff);
-  ap.sbuf = db.alloc<double>((size_t)soff);
-  ap.gtin = db.up(std::vector<double>(ctx.nbh_total, 0.0));
+  ap.gacc = db.up(std::vector<double>(ch.size(), 0.0));
   return ap;
 }
 
@@ -1754,13 +1751,16 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     if (dJ_du0 && (rc = hip_status(launch_mul_w(v, ag, static_cast<double*>(dJ_du0), A, C.stream), "W mu")))
       return rc;
     // gradients to the host: per level over the lines in order, then per layer
-    std::vector<double> gxp((size_t)s.ny * s.nz), gyp((size_t)ag.yparts * s.nx * s.nz), gt(nbh);
+    std::vector<double> gxp((size_t)s.ny * s.nz), gyp((size_t)ag.yparts * s.nx * s.nz), ga((size_t)ap.nchunk);
+    std::vector<RedChunk> chh((size_t)ap.nchunk);
     if ((rc = hip_status(hipMemcpyAsync(gxp.data(), ag.gx_part, sizeof(double) * gxp.size(), hipMemcpyDeviceToHost,
                                         C.stream), "copy")) ||
         (rc = hip_status(hipMemcpyAsync(gyp.data(), ag.gy_part, sizeof(double) * gyp.size(), hipMemcpyDeviceToHost,
                                         C.stream), "copy")) ||
-        (rc = hip_status(hipMemcpyAsync(gt.data(), ap.gtin, sizeof(double) * nbh, hipMemcpyDeviceToHost, C.stream),
-                         "copy")) ||
+        (rc = hip_status(hipMemcpyAsync(ga.data(), ap.gacc, sizeof(double) * ga.size(), hipMemcpyDeviceToHost,
+                                        C.stream), "copy")) ||
+        (rc = hip_status(hipMemcpyAsync(chh.data(), ap.chunk, sizeof(RedChunk) * chh.size(), hipMemcpyDeviceToHost,
+                                        C.stream), "copy")) ||
         (rc = hip_status(hipStreamSynchronize(C.stream), "sync")))
       return rc;
     for (int q = 0; q < nlay; ++q) dJ_dk[q] = 0.0;
@@ -1776,7 +1776,10 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
       }
       dJ_dk[lay] += tot;
     }
-    for (int b = 0; b < nbh; ++b) dJ_dTin[b] = gt[b];
+    // inlet gradient: the annulus chunks' accumulators of each borehole in chunk order
+    for (int b = 0; b < nbh; ++b) dJ_dTin[b] = 0.0;
+    for (size_t q = 0; q < chh.size(); ++q)
+      if (chh[q].part == 1) dJ_dTin[chh[q].bh] += ga[q];
     return GW_OK;
   } catch (const std::bad_alloc&) {
     return fail(GW_ENOMEM, "cannot allocate the adjoint's checkpoints and work fields");

@@ -153,7 +153,7 @@ struct AdjBh {
   int32_t turn_off;         // offset into turn_off_tab (nlev + 1 entries) of the turnaround transpose
   int32_t t0, nt;           // inner levels reduced for the turnaround (t0 .. t0+nt-1)
   int32_t i0, ni;           // annulus levels reduced for the inlet (i0 .. i0+ni-1)
-  int32_t red0, nred_in, nred_an, soff;  // chunks; offset of the annulus-level sums in sbuf
+  int32_t red0, nred_in, nred_an, pad;  // chunks: red0 .. red0+nred_in-1 inner, then nred_an annulus
 };
 // One pipe column for the adjoint kernels: its first cell, column class and
 // hx * hy (W at level k = hxy hz[k] rho[cls][k]).
@@ -171,9 +171,8 @@ struct AdjPipe {
   const AdjEnt* tent;       // turnaround entries (inner level, weight)
   const RedChunk* chunk;    // reduction chunks (part 0: inner/turn levels, 1: annulus/inlet levels)
   int32_t nchunk;
-  double* partial;          // chunk partial sums
-  double* sbuf;             // per-borehole annulus-level adjoint of the turnaround means
-  double* gtin;             // [nbh] accumulated d objective / d inlet temperature
+  double* partial;          // chunk partial sums (inner chunks, per turnaround level)
+  double* gacc;             // [nchunk] annulus chunks: accumulated d objective / d inlet temperature
 };
 hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap, double* mu, const double* seed,
                            hipStream_t st);
