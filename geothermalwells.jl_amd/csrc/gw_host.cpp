@@ -27,6 +27,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -35,6 +36,15 @@
 #include "gw_kernels.hpp"
 
 using namespace gw;
+
+// Scratch memory of gw_adi_adv_noalloc: the pipe reductions' chunk partials
+// and turnaround level means of a sub-step, in a fixed device array (one copy
+// per GPU, kept until the process ends) instead of the context's allocated
+// tables.  Host code never passes this name to a kernel: gw_create resolves
+// the current device's copy once (resolve_scratch) and the kernels receive
+// that address.
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
 
 namespace {
 
@@ -200,6 +210,7 @@ struct CTab {
   double* d_partial = nullptr;
   std::vector<Dep> h_dep;           // host copy of the departure table (adjoint)
   int nmeans = 0;
+  int npartial = 0;                 // doubles of d_partial
   int nrchunk = 0;
   int nagroup = 0;
 };
@@ -261,6 +272,11 @@ struct gw_ctx {
   double graph_dt = 0;
   // second field buffer of the fused pipeline (its halo kernels write out of place)
   void* buf2 = nullptr;
+  // device the context was created on, and that device's copy of the
+  // file-scope scratch array (gw_adi_adv_noalloc)
+  int device = 0;
+  double* scratch_dev = nullptr;
+  std::string scratch_err;  // why scratch_dev is NULL
 
   template <class T>
   T* up(const std::vector<T>& h) {
@@ -684,6 +700,7 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
     t.nagroup = (int)groups.size();
     std::vector<double> part(std::max(poff, 1), 0.0);
     t.d_partial = dev_upload(part);
+    t.npartial = poff;
   } catch (HipFail& e) {
     return fail(GW_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e.e));
   }
@@ -712,8 +729,16 @@ double stability_bound(double c3) {
 }
 
 // One advection sub-step (advection!, src/solver.jl:250-264) on phi.
-int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* jacc = nullptr) {
+// scratch (may be NULL): at least t->npartial + t->nmeans doubles of device
+// memory used for the reduction partials and level means instead of the
+// tables' own buffers.
+int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* jacc = nullptr,
+                    double* scratch_mem = nullptr) {
   DevView v = ctx.view(t);
+  if (scratch_mem) {
+    v.partial = scratch_mem;
+    v.means = scratch_mem + t->npartial;
+  }
   int rc = hip_status(launch_pipe_reduce(v, ctx.prec, phi, ctx.stream, jacc), "pipe_reduce");
   if (rc) return rc;
   if (ctx.host_inlet) {
@@ -742,24 +767,47 @@ int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* j
   return hip_status(launch_pipe_advect(v, ctx.prec, phi, ctx.stream), "pipe_advect");
 }
 
-int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
-  CTab* tab = nullptr;
+// ADI_and_ADV_callback! with the tables of c = dt/2 (scratch_mem: see
+// sub_step_advect).
+int adi_adv_tab(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, double* scratch_mem) {
   const double c = dt / 2;
-  int rc = build_ctab(ctx, c, tab);
-  if (rc) return rc;
   DevView v = ctx.view(tab);
   // explicit y  (src/solver.jl:444)
-  rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "explicit y");
+  int rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "explicit y");
   if (rc) return rc;
-  rc = sub_step_advect(ctx, tab, phi, t);  // :447
+  rc = sub_step_advect(ctx, tab, phi, t, nullptr, scratch_mem);  // :447
   if (rc) return rc;
   // implicit x then explicit x  (:450, :455)
   rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), phi, phi, c, 0.0, 0.0, ctx.stream), "implicit+explicit x");
   if (rc) return rc;
-  rc = sub_step_advect(ctx, tab, phi, t + dt / 2);  // :458
+  rc = sub_step_advect(ctx, tab, phi, t + dt / 2, nullptr, scratch_mem);  // :458
   if (rc) return rc;
   // implicit y  (:461)
   return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "implicit y");
+}
+
+int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
+  CTab* tab = nullptr;
+  const int rc = build_ctab(ctx, dt / 2, tab);
+  if (rc) return rc;
+  return adi_adv_tab(ctx, tab, phi, t, dt, nullptr);
+}
+
+// The current device's copy of `scratch`, resolved once per device.
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;
+
+int resolve_scratch(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(GW_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(dev, static_cast<double*>(p)).first;
+  }
+  *out = it->second;
+  return GW_OK;
 }
 
 // The z-step applies R(dt A_z) = 1 + z + z^2/2 + c3 z^3 at every ROCK2 degree:
@@ -964,6 +1012,13 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
   } catch (HipFail& e) {
     return fail(GW_EHIP, std::string("device set-up failed: ") + hipGetErrorString(e.e));
   }
+  // the no-allocation entry point's scratch on this device; a failure here
+  // only disables that entry point
+  if (hipGetDevice(&ctx->device) != hipSuccess) return fail(GW_EHIP, "hipGetDevice failed");
+  if (resolve_scratch(ctx->device, &ctx->scratch_dev) != GW_OK) {
+    ctx->scratch_dev = nullptr;
+    ctx->scratch_err = g_err;
+  }
   *out = ctx.release();
   return GW_OK;
 }
@@ -1000,6 +1055,26 @@ int gw_adi_adv(gw_ctx* ctx, void* phi, void* /*scratch*/, double t, double dt) {
   if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
   if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
   return adi_adv(*ctx, phi, t, dt);
+}
+
+int gw_adi_adv_noalloc(gw_ctx* ctx, void* phi, double t, double dt) {
+  if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
+  if (!(dt > 0)) return fail(GW_EINVAL, "dt must be positive");
+  if (ctx->host_inlet)
+    return fail(GW_EINVAL, "gw_adi_adv_noalloc: a host-evaluated inlet (GW_INLET_HOST) synchronises the stream "
+                           "every sub-step; use gw_adi_adv");
+  if (!ctx->scratch_dev) return fail(GW_EHIP, "gw_adi_adv_noalloc: no scratch on this device (" + ctx->scratch_err + ")");
+  const auto it = ctx->ctabs.find(dt / 2);
+  if (it == ctx->ctabs.end())
+    return fail(GW_ESTATE, "gw_adi_adv_noalloc: no tables for this dt yet; one allocating call with the same dt "
+                           "(gw_adi_adv, gw_step, gw_steps) builds them");
+  const CTab* tab = &it->second;
+  const size_t need = (size_t)tab->npartial + (size_t)tab->nmeans;
+  if (need > kNoAllocScratchDoubles)
+    return fail(GW_EINVAL, "gw_adi_adv_noalloc: the pipe reductions need " + std::to_string(need) +
+                               " doubles of scratch, more than its " + std::to_string(kNoAllocScratchDoubles) +
+                               " (8 MiB); use gw_adi_adv");
+  return adi_adv_tab(*ctx, tab, phi, t, dt, ctx->scratch_dev);
 }
 
 int gw_zstep(gw_ctx* ctx, void* phi, double dt) {

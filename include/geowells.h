@@ -120,6 +120,19 @@ int gw_rhs_z(gw_ctx *ctx, void *dphi, const void *phi, double t);
  * overwrites: it may be NULL and is not written. */
 int gw_adi_adv(gw_ctx *ctx, void *phi, void *scratch, double t, double dt);
 
+/* gw_adi_adv without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same kernels and results
+ * as gw_adi_adv; the pipe reductions' partial sums and turnaround level means
+ * live in a fixed 8 MiB device array per GPU (held until the process ends)
+ * instead of the context's tables.  Needs the context's tables for this dt,
+ * built by one earlier allocating call with the same dt (gw_adi_adv, gw_step,
+ * gw_steps): GW_ESTATE otherwise.  GW_EINVAL when the reductions need more
+ * than 2^20 doubles of scratch, or for a GW_INLET_HOST inlet (its host round
+ * trip synchronises).  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two streams, two threads) race. */
+int gw_adi_adv_noalloc(gw_ctx *ctx, void *phi, double t, double dt);
+
 /* The ROCK2 z-step of one step at its minimum degree, fused:
  * phi = R(dt*A_z) phi with R(z) = 1 + z + z^2/2 + c3 z^3 (the polynomial the
  * examples' ROCK2(eigen_est=eigen_estimator) applies, src/solver.jl:23-31,
