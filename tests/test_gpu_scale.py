@@ -149,7 +149,7 @@ def test_long_line_operators(n_target, op, direction, line_window):
     got = d.view(u.shape).cpu().numpy()
     err = np.abs(got - ref)
     i, j, k = np.unravel_index(np.argmax(err), err.shape)
-    assert H.rel_err(got, ref) <= 1e-10, (orc.shape, (i, j, k), float(err.max()))
+    assert H.rel_err(got, ref) <= 1e-10, (got.shape, (i, j, k), float(err.max()))
 
 
 @pytest.mark.parametrize("line_window", LINE_WINDOWS)
@@ -174,7 +174,7 @@ def test_fused_line_modes(n_target, mode, line_window):
     got = dst.view(u.shape).cpu().numpy()
     err = np.abs(got - ref)
     i, j, k = np.unravel_index(np.argmax(err), err.shape)
-    assert H.rel_err(got, ref) <= 1e-10, (orc.shape, (i, j, k), float(err.max()))
+    assert H.rel_err(got, ref) <= 1e-10, (got.shape, (i, j, k), float(err.max()))
 
 
 @pytest.mark.parametrize("line_window", LINE_WINDOWS + [None])
