@@ -8,13 +8,13 @@ this package is the host-side mirror of the reference's Julia interface.
 from . import _abi
 from .adjoint import outlet_gradient
 from .cache import (Cache, CallbackSet, DiscreteCallback, SavedValues, create_batch_cache, create_cache,
-                    get_simulation_callback)
+                    get_simulation_callback, load_checkpoint, save_checkpoint)
 from .grids import compute_domain, create_adaptive_grid_1d, create_uniform_gridz_with_borehole_depths
 from .physics import (AbstractBorehole, AbstractMaterialProperties, Borehole, ConstantInlet, CustomInlet,
                       HeatExchangerInlet, HomogenousMaterialProperties, PerWellInlet, StratifiedMaterialProperties,
                       initial_condition_thermal_gradient)
 from .solver import (ADI_and_ADV_callback_, ODEProblem, ROCK2, eigen_estimator, rhs_diffusion_z_,
-                     rock2_coefficients, solve)
+                     rock2_coefficients, solve, step_plan)
 
 LIB_PATH = _abi.LIB_PATH
 

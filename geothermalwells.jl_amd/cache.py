@@ -11,7 +11,7 @@ from .physics import (ConstantInlet, CustomInlet, HeatExchangerInlet, Homogenous
                       StratifiedMaterialProperties)
 
 __all__ = ["create_cache", "create_batch_cache", "Cache", "get_simulation_callback", "DiscreteCallback",
-           "CallbackSet", "SavedValues"]
+           "CallbackSet", "SavedValues", "save_checkpoint", "load_checkpoint"]
 
 DEFAULT_ROCK2_C3 = 1.0 / 16.0
 
@@ -287,11 +287,14 @@ def save_and_print_callback(saveat, print_every_n=1000, write_to_jld=False, data
     counter = [0]
 
     def save_host(u, t, integrator):
-        u_cpu = u.detach().to("cpu").numpy().astype(Float_used_to_save, copy=True)
         if write_to_jld:
+            # the file keeps the state's own precision (the reference writes
+            # u_cpu, src/cache.jl:154-161), with t and the outlet so a run can
+            # resume from it
             counter[0] += 1
-            np.save(os.path.join(data_folder_dir, f"checkpoint_{counter[0]}.npy"), u_cpu)
-        return u_cpu
+            save_checkpoint(os.path.join(data_folder_dir, f"checkpoint_{counter[0]}.npz"), u, t,
+                            cache=integrator.p, naccept=integrator.stats.naccept)
+        return u.detach().to("cpu").numpy().astype(Float_used_to_save, copy=True)
 
     if write_to_jld and not os.path.isdir(data_folder_dir):
         os.mkdir(data_folder_dir)
@@ -311,3 +314,64 @@ def get_simulation_callback(*, saveat, print_every_n=1000, write_to_jld=False, d
     adi = DiscreteCallback(lambda u, t, integ: True, ADI_and_ADV_callback_)
     adi.every_step_adi = True  # the fused driver runs it inside gw_steps
     return CallbackSet(adi, print_cb, save_cb), saved_values
+
+
+# ---------------------------------------------------------------------------
+# Checkpoint / resume.  The reference writes checkpoint_<n>.jld2 holding the
+# state (src/cache.jl:154-161) and leaves saving T_outlet -- and with it a
+# resume -- as a TODO (src/cache.jl:121).  A checkpoint here is an .npz with
+# the state at its own precision (the context's flat layout, each scenario's
+# (Nz, Ny, Nx) column-major field back to back), the time t, the step count
+# and the last outlet temperatures.  A run resumes by solving from the loaded
+# state over (t, t_end): with the same line-kernel variants (forced
+# line_window, or the same context) the resumed steps are bit-identical to
+# the uninterrupted ones.
+# ---------------------------------------------------------------------------
+CHECKPOINT_VERSION = 1
+
+
+def save_checkpoint(path, u, t, cache=None, naccept=None):
+    """Write state `u` (a tensor, any device) at its own dtype with time `t`;
+    with a cache also its T_outlet and scenario shapes."""
+    arr = u.detach().to("cpu").contiguous().numpy()
+    data = dict(version=np.int64(CHECKPOINT_VERSION), u=arr.reshape(-1), t=np.float64(t),
+                naccept=np.int64(-1 if naccept is None else naccept))
+    if cache is not None:
+        if arr.size != cache.state_elems:
+            raise ValueError("state size does not match the cache")
+        data["T_outlet"] = cache.T_outlet
+        data["shapes"] = np.array([cache.scenario_shape(q) for q in range(cache.n_scenarios)], dtype=np.int64)
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, **data)
+    os.replace(tmp, path)
+    return path
+
+
+class Checkpoint:
+    def __init__(self, u, t, naccept, T_outlet, shapes):
+        self.u, self.t, self.naccept, self.T_outlet, self.shapes = u, t, naccept, T_outlet, shapes
+
+
+def load_checkpoint(path, cache=None, device=None):
+    """Read a checkpoint written by save_checkpoint.  With a cache the state
+    is checked against its size, dtype and scenario shapes and placed on its
+    device; otherwise on `device` (default: CPU)."""
+    with np.load(path, allow_pickle=False) as f:
+        if int(f["version"]) != CHECKPOINT_VERSION:
+            raise ValueError(f"{path}: unknown checkpoint version {int(f['version'])}")
+        u = np.array(f["u"])
+        t = float(f["t"])
+        naccept = int(f["naccept"])
+        T_out = np.array(f["T_outlet"]) if "T_outlet" in f else None
+        shapes = [tuple(int(v) for v in r) for r in f["shapes"]] if "shapes" in f else None
+    ut = torch.from_numpy(u)
+    if cache is not None:
+        if ut.dtype != cache.dtype or ut.numel() != cache.state_elems:
+            raise ValueError(f"{path}: the state is {ut.numel()} x {ut.dtype}, the cache needs "
+                             f"{cache.state_elems} x {cache.dtype}")
+        if shapes is not None and shapes != [cache.scenario_shape(q) for q in range(cache.n_scenarios)]:
+            raise ValueError(f"{path}: scenario shapes differ from the cache's")
+        ut = ut.to(cache.backend)
+    elif device is not None:
+        ut = ut.to(device)
+    return Checkpoint(ut.contiguous(), t, naccept, T_out, shapes)

@@ -8,7 +8,7 @@ from . import _abi
 from .cache import CallbackSet, DiscreteCallback, SavingCallback
 
 __all__ = ["rhs_diffusion_z_", "ADI_and_ADV_callback_", "eigen_estimator", "eigen_estimator_pre_calculation",
-           "ODEProblem", "ROCK2", "solve", "rock2_coefficients"]
+           "ODEProblem", "ROCK2", "solve", "rock2_coefficients", "step_plan"]
 
 
 def _ptr(cache, t):
@@ -130,6 +130,36 @@ def _fused_driver_ok(alg, cbs):
     return known and n_adi == 1
 
 
+def step_plan(tspan, dt, saveat=()):
+    """The fixed-step plan of solve(adaptive=false): the number of full steps
+    of dt, and the length of a last shorter step (0.0 if none) when tspan is
+    not a whole number of steps -- OrdinaryDiffEq shortens the step that would
+    pass the end of tspan so the run ends on it.  Save times inside tspan must
+    be step times: the reference's SavingCallback interpolates between steps
+    (its dense output), which this mirror does not reproduce, so an unaligned
+    time raises instead of being dropped."""
+    t0, t1 = float(tspan[0]), float(tspan[1])
+    dt = float(dt)
+    if not dt > 0 or not t1 >= t0:
+        raise ValueError("need dt > 0 and tspan[1] >= tspan[0]")
+    span = t1 - t0
+    n = int(math.floor(span / dt + 1e-9))
+    rem = span - n * dt
+    last = rem if rem > 1e-9 * dt else 0.0
+    tol = lambda x: 1e-9 * max(1.0, abs(x))
+    for st in saveat:
+        st = float(st)
+        if st < t0 - tol(st) or st > t1 + tol(st):
+            continue
+        k = int(round((st - t0) / dt))
+        on_step = 0 <= k <= n and abs(t0 + k * dt - st) <= tol(st)
+        if not on_step and not (last and abs(st - t1) <= tol(st)):
+            raise ValueError(f"saveat time {st} is not a step time (t0 + k*dt with dt = {dt}): the reference "
+                             f"interpolates such times between steps, which this solver does not; choose save "
+                             f"times on the step grid")
+    return n, last
+
+
 def _next_event(cbs, n, nsteps, t0, dt):
     """First step index > n at which a printing or saving callback acts."""
     nxt = nsteps
@@ -154,11 +184,14 @@ def solve(prob, alg, *, callback=None, dt, adaptive=False, save_everystep=False,
         raise NotImplementedError("only adaptive=false is supported (as in the reference examples)")
     integ = Integrator(prob, alg, float(dt))
     cbs = callback.callbacks if isinstance(callback, CallbackSet) else ([callback] if callback else [])
+    t0, t1 = prob.tspan
+    saveat = [st for cb in cbs if isinstance(cb, SavingCallback) for st in cb.saveat]
+    nsteps, dt_last = step_plan(prob.tspan, dt, saveat)
+    if nsteps > maxiters:
+        nsteps, dt_last = maxiters, 0.0
     for cb in cbs:
         if isinstance(cb, SavingCallback):
             cb.maybe_save(integ)
-    t0, t1 = prob.tspan
-    nsteps = min(int(round((t1 - t0) / dt)), maxiters)
     if _fused_driver_ok(alg, cbs):
         p = integ.p
         n = 0
@@ -169,21 +202,31 @@ def solve(prob, alg, *, callback=None, dt, adaptive=False, save_everystep=False,
             integ.t = t0 + n * dt
             integ.tprev = integ.t - dt
             integ.stats.naccept = n
-            for cb in cbs:
-                if isinstance(cb, SavingCallback):
-                    cb.maybe_save(integ)
-                elif not getattr(cb, "every_step_adi", False) and cb.condition(integ.u, integ.t, integ):
-                    cb.affect(integ)
+            _after_step(integ, cbs, fused=True)
+        if dt_last:
+            _abi.check(p._lib.gw_steps(p.ctx, _ptr(p, integ.u), float(integ.t), float(dt_last), 1))
+            integ.tprev, integ.t = integ.t, t1
+            integ.stats.naccept += 1
+            _after_step(integ, cbs, fused=True)
         return integ
-    for n in range(nsteps):
+    for n in range(nsteps + (1 if dt_last else 0)):
         integ.tprev = integ.t
+        if n == nsteps:
+            integ.dt = dt_last
         _rock2_step(integ)
-        integ.t = t0 + (n + 1) * dt
+        integ.t = t0 + (n + 1) * dt if n < nsteps else t1
         integ.stats.naccept += 1
-        for cb in cbs:
-            if isinstance(cb, DiscreteCallback):
-                if cb.condition(integ.u, integ.t, integ):
-                    cb.affect(integ)
-            elif isinstance(cb, SavingCallback):
-                cb.maybe_save(integ)
+        _after_step(integ, cbs, fused=False)
     return integ
+
+
+def _after_step(integ, cbs, fused):
+    """Callbacks after a step, in order: DiscreteCallback affects (the fused
+    driver has already applied the ADI/advection one), saving at save times."""
+    for cb in cbs:
+        if isinstance(cb, SavingCallback):
+            cb.maybe_save(integ)
+        elif fused and getattr(cb, "every_step_adi", False):
+            continue
+        elif cb.condition(integ.u, integ.t, integ):
+            cb.affect(integ)

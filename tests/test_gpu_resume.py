@@ -1,0 +1,132 @@
+"""Resume from a checkpoint: a run split at a checkpoint file (state at its own
+precision, t) and continued in a fresh context equals the uninterrupted run
+bit for bit when both contexts use the same line-kernel variants; through
+solve() with the saving callback's checkpoint files; and at the 3x3 array's
+full size within one context."""
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _kw(w):
+    return {k: w[k] for k in ("gridx", "gridy", "gridz", "materials", "boreholes", "inlet_model")}
+
+
+def _ic(g, cache, w):
+    import torch
+    gz = np.asarray(w["gridz"], dtype=np.float64)
+    col = np.float64(w["ic"]["T_surface"]) + np.float64(w["ic"]["gradient"]) * gz
+    u = cache.new_state()
+    cache.scenario_view(u, 0).copy_(torch.from_numpy(col).to(u.dtype).expand(*cache.scenario_shape(0)))
+    return u
+
+
+def _steps(g, cache, u, t0, dt, n):
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), float(t0), float(dt), int(n)))
+    cache.sync()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("window", [64, 32])
+def test_split_run_equals_uninterrupted(tmp_path, precision, window):
+    import torch
+    g = H.gw()
+    w = H.small_case(nbh=2, layered=True, inlet="perwell")
+    dt, n, k = w["dt"], 10, 4
+    a = g.create_cache(precision=precision, line_window=window, **_kw(w))
+    u_a = _ic(g, a, w)
+    _steps(g, a, u_a, 0.0, dt, n)
+    b = g.create_cache(precision=precision, line_window=window, **_kw(w))
+    u_b = _ic(g, b, w)
+    _steps(g, b, u_b, 0.0, dt, k)
+    path = os.path.join(tmp_path, "ck.npz")
+    g.save_checkpoint(path, u_b, k * dt, cache=b, naccept=k)
+    del b, u_b
+    c = g.create_cache(precision=precision, line_window=window, **_kw(w))
+    ck = g.load_checkpoint(path, cache=c)
+    assert ck.t == k * dt and ck.naccept == k and ck.u.dtype == c.dtype
+    _steps(g, c, ck.u, ck.t, dt, n - k)
+    assert torch.equal(ck.u, u_a)
+    assert np.array_equal(c.T_outlet, a.T_outlet)
+
+
+def test_solve_resumes_from_callback_checkpoint(tmp_path):
+    """get_simulation_callback(write_to_jld=true) writes checkpoint_<n>.npz at
+    every save time; solve() from the second one over the rest of tspan ends
+    where the uninterrupted solve ends."""
+    import torch
+    g = H.gw()
+    w = H.small_case(inlet="hx")
+    dt, n = w["dt"], 9
+    tspan = (0.0, n * dt)
+
+    def run(u0, span, folder=None):
+        cache = g.create_cache(line_window=64, **_kw(w))
+        cb, saved = g.get_simulation_callback(saveat=[0.0, 3 * dt, n * dt], print_every_n=10 ** 6,
+                                              write_to_jld=folder is not None, data_folder_dir=folder or "")
+        prob = g.ODEProblem(g.rhs_diffusion_z_, u0.to(cache.backend), span, cache)
+        integ = g.solve(prob, g.ROCK2(eigen_est=g.eigen_estimator, fused=True), callback=cb, dt=dt, adaptive=False)
+        return cache, integ, saved
+
+    base = g.create_cache(line_window=64, **_kw(w))
+    u0 = _ic(g, base, w)
+    folder = os.path.join(tmp_path, "ck")
+    ca, ia, saved = run(u0.clone(), tspan, folder)
+    assert len(saved.t) == 3 and sorted(os.listdir(folder)) == [f"checkpoint_{q}.npz" for q in (1, 2, 3)]
+    assert saved.saveval[1].dtype == np.float32
+    cache = g.create_cache(line_window=64, **_kw(w))
+    ck = g.load_checkpoint(os.path.join(folder, "checkpoint_2.npz"), cache=cache)
+    assert ck.t == 3 * dt and ck.u.dtype == torch.float64
+    cb, _ = g.get_simulation_callback(saveat=[], print_every_n=10 ** 6)
+    prob = g.ODEProblem(g.rhs_diffusion_z_, ck.u, (ck.t, n * dt), cache)
+    ib = g.solve(prob, g.ROCK2(eigen_est=g.eigen_estimator, fused=True), callback=cb, dt=dt, adaptive=False)
+    assert ib.t == ia.t == n * dt
+    assert torch.equal(ib.u, ia.u)
+    assert np.array_equal(cache.T_outlet, ca.T_outlet)
+    # the last checkpoint holds the final state at full precision
+    last = g.load_checkpoint(os.path.join(folder, "checkpoint_3.npz"), cache=cache)
+    assert torch.equal(last.u, ia.u) and np.array_equal(last.T_outlet, ca.T_outlet)
+
+
+def test_shortened_last_step_ends_on_tspan():
+    """tspan not a whole number of steps: the last step is shortened (as
+    OrdinaryDiffEq does with adaptive=false) and equals one separate step of
+    that length."""
+    import torch
+    g = H.gw()
+    w = H.small_case()
+    dt = w["dt"]
+    cache = g.create_cache(line_window=64, **_kw(w))
+    u = _ic(g, cache, w)
+    cb, _ = g.get_simulation_callback(saveat=[], print_every_n=10 ** 6)
+    prob = g.ODEProblem(g.rhs_diffusion_z_, u, (0.0, 3 * dt + 25.0), cache)
+    integ = g.solve(prob, g.ROCK2(eigen_est=g.eigen_estimator, fused=True), callback=cb, dt=dt, adaptive=False)
+    assert integ.t == 3 * dt + 25.0
+    ref_cache = g.create_cache(line_window=64, **_kw(w))
+    v = _ic(g, ref_cache, w)
+    _steps(g, ref_cache, v, 0.0, dt, 3)
+    _steps(g, ref_cache, v, 3 * dt, 25.0, 1)
+    assert torch.equal(integ.u, v)
+
+
+def test_array_3x3_split_in_one_context():
+    """Full size: 5 steps in one call against 2 + 3 in two calls of the same
+    context (the fused pipeline restarts at the call boundary)."""
+    import torch
+    from geothermalwells_jl_amd import workloads
+    g = H.gw()
+    w = workloads.c1()
+    cache = g.create_cache(**_kw(w))
+    u_a = _ic(g, cache, w)
+    u_b = u_a.clone()
+    _steps(g, cache, u_a, 0.0, w["dt"], 5)
+    T_a = cache.T_outlet
+    _steps(g, cache, u_b, 0.0, w["dt"], 2)
+    _steps(g, cache, u_b, 2 * w["dt"], w["dt"], 3)
+    assert torch.equal(u_a, u_b)
+    assert np.array_equal(cache.T_outlet, T_a)
