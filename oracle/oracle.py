@@ -73,6 +73,8 @@ def lib():
         L.or_objective_dual.restype = ctypes.c_int
         L.or_objective_dual.argtypes = [ctypes.POINTER(OrScenario), _dp, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        L.or_rock2_minimum_degree.restype = ctypes.c_int
+        L.or_rock2_minimum_degree.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.or_set_threads.argtypes = [ctypes.c_int]
         L.or_max_threads.restype = ctypes.c_int
         _lib = L
@@ -206,6 +208,10 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(lib().or_last_error().decode())
         return J.value, dJ[:n.value].copy()
+
+    def rock2_minimum_degree(self, dt):
+        """True when ROCK2's degree rule keeps its 3-stage member for this dt."""
+        return bool(lib().or_rock2_minimum_degree(self.h, float(dt)))
 
     def run(self, u, t0, dt, nsteps, c3):
         """Returns (u_final, outlet history [nsteps, nbh], seconds)."""
