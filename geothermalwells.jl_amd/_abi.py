@@ -73,6 +73,13 @@ SIGNATURES = {
     "gw_implicit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                    ctypes.c_int32]),
     "gw_advect": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "gw_compute_domain": (ctypes.c_int, [ctypes.POINTER(GwBorehole), ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, c_double_p]),
+    "gw_uniform_gridz": (ctypes.c_int64, [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.POINTER(GwBorehole), ctypes.c_int32, c_double_p, ctypes.c_int64]),
+    "gw_adaptive_grid_1d": (ctypes.c_int64, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.POINTER(GwBorehole), ctypes.c_int32,
+                                             ctypes.c_int32, c_double_p, ctypes.c_int64]),
     "gw_state_elems": (ctypes.c_int64, [ctypes.c_void_p]),
     "gw_scenario_offset": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int32]),
     "gw_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),

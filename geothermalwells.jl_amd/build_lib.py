@@ -35,6 +35,8 @@ def _run_all(cmds):
         raise subprocess.CalledProcessError(1, failed[0])
 
 
+# host units: set-up, C ABI, grid generators (no floating-point contraction)
+HOST_UNITS = ["gw_host.cpp", "gw_grids.cpp"]
 KERNEL_UNITS = ["gw_kernels.hip", "gw_line_w64_f64.hip", "gw_line_w64_f32.hip", "gw_line_w32_f64.hip",
                 "gw_line_w32_f32.hip", "gw_line_adj_w64.hip", "gw_line_adj_w32.hip", "gw_adjoint.hip"]
 
@@ -45,14 +47,15 @@ def build(verbose=True):
     os.makedirs(objdir, exist_ok=True)
     common = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
               "-I", CSRC]
-    host_obj = os.path.join(objdir, "gw_host.o")
+    host_objs = [os.path.join(objdir, u.replace(".cpp", ".o")) for u in HOST_UNITS]
     kern_objs = [os.path.join(objdir, u.replace(".hip", ".o")) for u in KERNEL_UNITS]
-    cmds = [[hipcc, *common, "-ffp-contract=off", "-c", os.path.join(CSRC, "gw_host.cpp"), "-o", host_obj]]
+    cmds = [[hipcc, *common, "-ffp-contract=off", "-c", os.path.join(CSRC, u), "-o", o]
+            for u, o in zip(HOST_UNITS, host_objs)]
     cmds += [[hipcc, *common, *DEVICE_FLAGS, "-c", os.path.join(CSRC, u), "-o", o]
              for u, o in zip(KERNEL_UNITS, kern_objs)]
     _run_all(cmds)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, host_obj, *kern_objs])
+    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *host_objs, *kern_objs])
     os.replace(tmp, LIB)
     return LIB
 

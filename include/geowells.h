@@ -207,6 +207,24 @@ int gw_adjoint(gw_ctx *ctx, const void *phi0, double t0, double dt, int32_t nste
  * counterpart in the reference (it has no adjoint). */
 int gw_adjoint_op(gw_ctx *ctx, int32_t op, void *out, const void *in, const void *x, double dt);
 
+/* Grid generators (src/grids.jl), host only: no context, no GPU needed.
+ * The two grid functions write at most `cap` values to `out` (which may be
+ * NULL) and return the grid's length, or a negative gw_status; Julia's float
+ * ranges are reproduced exactly (decimal endpoints as exact rationals, each
+ * element rounded once).
+ * compute_domain (src/grids.jl:8-18): out[6] = xmin, xmax, ymin, ymax, zmin, zmax. */
+int gw_compute_domain(const gw_borehole *bh, int32_t n_bh, double buffer_x, double buffer_y, double buffer_z,
+                      double *out);
+/* create_uniform_gridz_with_borehole_depths (src/grids.jl:29-48): zmin:dz:zmax
+ * plus every borehole depth, sorted, duplicates removed. */
+int64_t gw_uniform_gridz(double zmin, double zmax, double dz, const gw_borehole *bh, int32_t n_bh, double *out,
+                         int64_t cap);
+/* create_adaptive_grid_1d (src/grids.jl:80-147), direction 0 = x, 1 = y: fine
+ * uniform spacing across each borehole's backfill diameter, geometric growth
+ * (capped at dx_max) towards the domain ends and between boreholes. */
+int64_t gw_adaptive_grid_1d(double xmin, double xmax, double dx_fine, double growth_factor, double dx_max,
+                            const gw_borehole *bh, int32_t n_bh, int32_t direction, double *out, int64_t cap);
+
 /* Introspection (cache fields the reference exposes). */
 int64_t gw_state_elems(gw_ctx *ctx);
 int64_t gw_scenario_offset(gw_ctx *ctx, int32_t scen);

@@ -129,8 +129,11 @@ class Oracle:
         self.shape = (self.nx, self.ny, self.nz)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_destroy(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_destroy(self.h)
+            except Exception:  # interpreter teardown
+                pass
             self.h = None
 
     def _chk(self, rc):
