@@ -143,8 +143,14 @@ def c3():
                 inlet_model=ConstantInlet(20.0), dt=300.0, nsteps=8640, ic=_ic())
 
 
-def c4():
-    """5x5 array at 40 m spacing, 601 x 601 x 2002, inlets U(10,25), dt 600 s, 1 year."""
+def c4(dt=600.0):
+    """5x5 array at 40 m spacing, 601 x 601 x 2002, inlets U(10,25), dt 600 s, 1 year.
+
+    At dt = 600 s the reference's step sequence is not stable on this grid: a
+    mode in the steel plug below each borehole bottom (z in (h, h + 3],
+    4 mm cells at the casing radii) grows by about 4% per step and dominates
+    the field after about 200 steps, in the oracle as on the GPU.  dt = 300 s
+    (the same year in 105,120 steps) stays bounded."""
     rng = SplitMix64(4)
     pos = [-80.0, -40.0, 0.0, 40.0, 80.0]
     bhs = tuple(coaxial(x, y) for x in pos for y in pos)
@@ -154,7 +160,7 @@ def c4():
     gy = create_adaptive_grid_1d(xmin=-150.0, xmax=150.0, direction="y", **kw)
     gz = create_uniform_gridz_with_borehole_depths(zmin=0.0, zmax=2100.0, dz=1.05, boreholes=bhs)
     return dict(name="c4_5x5_array", gridx=gx, gridy=gy, gridz=gz, materials=homogeneous(), boreholes=bhs,
-                inlet_model=PerWellInlet(T_in), dt=600.0, nsteps=52560, ic=_ic())
+                inlet_model=PerWellInlet(T_in), dt=float(dt), nsteps=int(round(365 * 86400 / dt)), ic=_ic())
 
 
 def scenario_kwargs(w):

@@ -1,0 +1,302 @@
+"""GPU parity at the full sizes of the batched (64 ragged scenarios), adjoint
+(121 x 121 x 602, 20 layers) and mixed-precision (5x5 array, 601 x 601 x
+2002) configurations, through the C ABI:
+
+* batched FP64 equals every scenario run in a context of its own; the three
+  smallest scenarios match the oracle; batched FP32 stays within 1e-3 K of
+  batched FP64;
+* the adjoint's gradient matches the oracle's dual-number derivative over a
+  two-step prefix at full size, and central differences of the GPU forward
+  run (h = 1e-6 relative) over 128 steps checkpointed every 64;
+* one FP64 step of the 5x5 array matches the oracle (1e-10 field, 1e-6 K
+  outlet); FP32 state stays within 1e-3 K of FP64 after five days at
+  dt = 300 s; on a small analog of its borehole bottom the GPU follows the
+  oracle through the growing mode the reference method has at dt = 600 s.
+"""
+import functools
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+FIELD_TOL = 1e-10
+OUTLET_TOL = 1e-6
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def W():
+    from geothermalwells_jl_amd import workloads
+    return workloads
+
+
+def _ic_state(cache, ws):
+    """The linear initial field of every scenario in the context's state."""
+    torch = _torch()
+    u = cache.new_state()
+    for s, w in enumerate(ws):
+        gz = np.asarray(w["gridz"], dtype=np.float64)
+        col = np.float64(w["ic"]["T_surface"]) + np.float64(w["ic"]["gradient"]) * gz
+        cache.scenario_view(u, s).copy_(torch.from_numpy(col).to(u.dtype).expand(*cache.scenario_shape(s)))
+    return u
+
+
+def _steps(cache, u, dt, n, t0=0.0):
+    g = H.gw()
+    g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), float(t0), float(dt), int(n)))
+    cache.sync()
+
+
+@functools.lru_cache(maxsize=None)
+def _c2():
+    return W().c2()
+
+
+# ---------------------------------------------------------------------------
+# 64 ragged scenarios
+# ---------------------------------------------------------------------------
+def test_batched_64_scenarios_equal_single_runs_and_oracle():
+    g = H.gw()
+    ws = _c2()
+    dt, n = ws[0]["dt"], 2
+    batch = g.create_batch_cache([W().scenario_kwargs(w) for w in ws], precision="f64")
+    ub = _ic_state(batch, ws)
+    _steps(batch, ub, dt, n)
+    Tb = batch.T_outlet
+    worst = 0.0
+    for s, w in enumerate(ws):
+        one = g.create_cache(precision="f64", **W().scenario_kwargs(w))
+        u1 = _ic_state(one, [w])
+        _steps(one, u1, dt, n)
+        got = batch.scenario_view(ub, s)
+        err = H.rel_err(got.cpu().numpy(), u1.view(got.shape).cpu().numpy())
+        worst = max(worst, err)
+        assert err <= 1e-12, (s, got.shape, err)
+        assert abs(Tb[s] - one.T_outlet[0]) <= 1e-12 * abs(Tb[s]), s
+        del one, u1
+    print(f"batched vs single: worst field rel diff {worst:.3e}", flush=True)
+    H.O().set_threads(min(16, os.cpu_count() or 1))
+    small = sorted(range(len(ws)), key=lambda s: W().cells(ws[s]))[:3]
+    for s in small:
+        w = ws[s]
+        orc = H.oracle_for(w)
+        u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+        ref, hist, _ = orc.run(u0, 0.0, dt, n, batch.rock2_c3)
+        got = batch.scenario_view(ub, s).cpu().numpy()
+        err = H.rel_err(got, ref)
+        print(f"scenario {s} {got.shape}: field rel err {err:.3e}, outlet err {abs(Tb[s] - hist[-1][0]):.3e} K",
+              flush=True)
+        assert err <= FIELD_TOL, (s, err)
+        assert abs(Tb[s] - hist[-1][0]) <= OUTLET_TOL, s
+
+
+def test_batched_64_scenarios_fp32_outlet_error():
+    """Batched FP32 state against batched FP64 after 720 steps (one day)."""
+    torch = _torch()
+    g = H.gw()
+    ws = _c2()
+    dt, n = ws[0]["dt"], 720
+    outs, fin = {}, {}
+    for prec in ("f64", "f32"):
+        cache = g.create_batch_cache([W().scenario_kwargs(w) for w in ws], precision=prec)
+        u = _ic_state(cache, ws)
+        _steps(cache, u, dt, n)
+        outs[prec] = cache.T_outlet
+        fin[prec] = bool(torch.isfinite(u).all().item())
+        del cache, u
+    err = np.abs(outs["f32"] - outs["f64"])
+    print(f"C2 FP32 vs FP64 after {n} steps: max outlet err {err.max():.3e} K", flush=True)
+    assert fin["f64"] and fin["f32"]
+    assert err.max() <= 1e-3
+
+
+# ---------------------------------------------------------------------------
+# layered single borehole: the adjoint at full size
+# ---------------------------------------------------------------------------
+def _c3_setup():
+    g = H.gw()
+    w = W().c3()
+    cache = g.create_cache(precision="f64", **W().scenario_kwargs(w))
+    u0 = _ic_state(cache, [w])
+    return g, w, cache, u0
+
+
+def test_adjoint_full_size_matches_dual_oracle():
+    """Two steps from the linear field: J and every derivative against the
+    oracle's forward-mode derivative of its own step sequence."""
+    g, w, cache, u0 = _c3_setup()
+    H.O().set_threads(min(16, os.cpu_count() or 1))
+    J, dk, dT = g.outlet_gradient(cache, u0, 0.0, w["dt"], 2, checkpoint_every=64)
+    orc = H.oracle_for(w)
+    uo = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+    Jo, dJo = orc.objective_dual(uo, 0.0, w["dt"], 2, cache.rock2_c3)
+    nk = len(dk)
+    print(f"C3 2-step: J {J:.15f} vs {Jo:.15f}; max |dk| {np.abs(dk).max():.3e}, "
+          f"dk diff {np.abs(dk - dJo[:nk]).max():.3e}, dT {dT.sum():.12e} vs {dJo[nk]:.12e}", flush=True)
+    assert abs(J - Jo) <= 1e-9 * abs(Jo)
+    assert np.abs(dk - dJo[:nk]).max() <= 1e-7 * np.abs(dJo[:nk]).max()
+    assert abs(dT.sum() - dJo[nk]) <= 1e-7 * abs(dJo[nk])
+
+
+def _objective(g, cache, u0, dt, n):
+    """J = mean over steps of the mean outlet by forward steps alone."""
+    u = u0.clone()
+    acc = 0.0
+    for s in range(n):
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), s * dt, dt, 1))
+        acc += float(cache.T_outlet.mean())
+    return acc / n
+
+
+def test_adjoint_full_size_central_differences():
+    """128 steps checkpointed every 64: dJ/dT_in and the two largest dJ/dk
+    against central differences of the GPU forward run (h = 1e-6 relative)."""
+    g, w, cache, u0 = _c3_setup()
+    dt, n, h = w["dt"], 128, 1e-6
+    J, dk, dT = g.outlet_gradient(cache, u0, 0.0, dt, n, checkpoint_every=64)
+    J_fwd = _objective(g, cache, u0, dt, n)
+    assert abs(J - J_fwd) <= 1e-12 * abs(J)
+    kw = W().scenario_kwargs(w)
+    mats = kw["materials"]
+
+    def J_of(materials=mats, T_in=None):
+        k2 = dict(kw, materials=materials)
+        if T_in is not None:
+            k2["inlet_model"] = g.ConstantInlet(T_in)
+        return _objective(g, g.create_cache(precision="f64", **k2), u0, dt, n)
+
+    def layered(q, f):
+        kl = list(mats.k_rock_layers)
+        kl[q] *= f
+        return g.StratifiedMaterialProperties(kl, mats.rho_c_rock_layers, mats.layer_depths, mats.k_water,
+                                              mats.rho_c_water, mats.k_steel, mats.rho_c_steel, mats.k_insulating,
+                                              mats.rho_c_insulating, mats.k_backfill, mats.rho_c_backfill)
+
+    T0 = kw["inlet_model"].T_const
+    fd_T = (J_of(T_in=T0 * (1 + h)) - J_of(T_in=T0 * (1 - h))) / (2 * h * T0)
+    rel = abs(dT.sum() - fd_T) / abs(fd_T)
+    print(f"C3 128 steps: dJ/dT_in {dT.sum():.12e} vs FD {fd_T:.12e} (rel {rel:.2e})", flush=True)
+    assert rel <= 1e-5
+    for q in np.argsort(-np.abs(dk))[:2]:
+        k0 = mats.k_rock_layers[q]
+        fd = (J_of(layered(int(q), 1 + h)) - J_of(layered(int(q), 1 - h))) / (2 * h * k0)
+        rel = abs(dk[q] - fd) / abs(fd)
+        print(f"  dJ/dk_{q + 1} {dk[q]:.12e} vs FD {fd:.12e} (rel {rel:.2e})", flush=True)
+        assert rel <= 1e-5
+
+
+# ---------------------------------------------------------------------------
+# 5x5 array, 601 x 601 x 2002
+# ---------------------------------------------------------------------------
+@pytest.mark.timeout(900)
+def test_array_5x5_step_matches_oracle():
+    """One FP64 step from the linear field plus seeded noise (a linear field
+    alone has A_x u = A_y u = 0 and would leave the x/y operators untested)."""
+    torch = _torch()
+    g = H.gw()
+    w = W().c4()
+    cache = g.create_cache(precision="f64", **W().scenario_kwargs(w))
+    c3 = cache.rock2_c3
+    H.O().set_threads(min(16, os.cpu_count() or 1))
+    orc = H.oracle_for(w)
+    u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+    u0 += np.random.default_rng(4).standard_normal(u0.shape)
+    u = torch.from_numpy(u0).cuda().reshape(-1).contiguous()
+    _steps(cache, u, w["dt"], 1)
+    T = cache.T_outlet
+    got = u.view(u0.shape).cpu().numpy()
+    del u, cache
+    torch.cuda.empty_cache()
+    print("C4: GPU step done; oracle z-step", flush=True)
+    import time
+    t0 = time.perf_counter()
+    ref = orc.rock2_step(u0, w["dt"], c3)
+    del u0
+    print(f"C4: oracle z-step {time.perf_counter() - t0:.1f} s; oracle ADI/advection", flush=True)
+    ref, T_ref = orc.adi_adv(ref, w["dt"], w["dt"])
+    err = H.rel_err(got, ref)
+    terr = float(np.abs(T - T_ref).max())
+    print(f"C4 1 step: field rel err {err:.3e}, outlet err {terr:.3e} K (oracle {time.perf_counter() - t0:.1f} s)",
+          flush=True)
+    assert err <= FIELD_TOL
+    assert terr <= OUTLET_TOL
+
+
+def test_array_5x5_fp32_outlet_error():
+    """FP32 state (FP64 elimination and reductions) against the FP64 run of
+    identical inputs after 1440 steps (five days) at dt = 300 s; the field
+    stays in the reference integration test's physical bounds
+    (test/test_integration.jl:27-29)."""
+    torch = _torch()
+    g = H.gw()
+    w = W().c4(dt=300.0)
+    n = 1440
+    outs, fin, rng = {}, {}, {}
+    for prec in ("f64", "f32"):
+        cache = g.create_cache(precision=prec, **W().scenario_kwargs(w))
+        u = _ic_state(cache, [w])
+        _steps(cache, u, w["dt"], n)
+        outs[prec] = cache.T_outlet
+        fin[prec] = bool(torch.isfinite(u).all().item())
+        rng[prec] = tuple(float(v) for v in torch.aminmax(u))
+        del cache, u
+        torch.cuda.empty_cache()
+    err = np.abs(outs["f32"] - outs["f64"])
+    print(f"C4 (dt 300 s) FP32 vs FP64 after {n} steps: max outlet err {err.max():.3e} K; field ranges {rng}",
+          flush=True)
+    assert fin["f64"] and fin["f32"]
+    assert err.max() <= 1e-3
+    for lo, hi in rng.values():
+        assert -50 < lo and hi < 200
+
+
+def _bottom_analog(h):
+    """One borehole with the 5x5 array's x/y grid rules (4 mm cells at the
+    pipe walls) and z spacing (1.05 m, the depth inserted as in
+    create_uniform_gridz_with_borehole_depths), 200 m deep: 111 x 111 x 202."""
+    g = H.gw()
+    bh = W().coaxial(0.0, 0.0, h=h)
+    kw = dict(dx_fine=0.004, growth_factor=1.25, dx_max=5.0, boreholes=(bh,))
+    gx = g.create_adaptive_grid_1d(xmin=-10.0, xmax=10.0, direction="x", **kw)
+    gz = g.create_uniform_gridz_with_borehole_depths(zmin=0.0, zmax=210.0, dz=1.05, boreholes=(bh,))
+    return dict(gridx=gx, gridy=gx.copy(), gridz=gz, materials=W().homogeneous(), boreholes=(bh,),
+                inlet_model=g.ConstantInlet(15.0), dt=600.0, ic=dict(T_surface=2.29, gradient=0.035))
+
+
+@pytest.mark.timeout(600)
+def test_array_5x5_bottom_analog_growth_matches_oracle():
+    """At the 5x5 array's dt = 600 s the reference step sequence is unstable
+    on these grids: a mode in the steel plug below the borehole bottom grows
+    by about 4% per step.  The GPU follows the oracle through it (field within
+    1e-10 of the oracle's maximum after 264 steps, when the mode dominates),
+    so the growth is the reference method's, not the kernels'."""
+    torch = _torch()
+    g = H.gw()
+    w = _bottom_analog(200.3)
+    cache = g.create_cache(**W().scenario_kwargs(w))
+    H.O().set_threads(min(16, os.cpu_count() or 1))
+    orc = H.oracle_for(w)
+    u0 = orc.initial_condition(2.29, 0.035)
+    u = torch.from_numpy(u0).cuda().reshape(-1).contiguous()
+    ref = u0
+    dev = []
+    for chunk in range(11):
+        _steps(cache, u, w["dt"], 24, t0=chunk * 24 * w["dt"])
+        ref, hist, _ = orc.run(ref, chunk * 24 * w["dt"], w["dt"], 24, cache.rock2_c3)
+        got = u.view(u0.shape).cpu().numpy()
+        below = np.asarray(w["gridz"]) > 200.3
+        dev.append(float(np.abs(ref - u0)[:, :, below].max()))
+        err = H.rel_err(got, ref)
+        print(f"analog step {24 * (chunk + 1)}: GPU vs oracle rel err {err:.3e}, "
+              f"max|u - u0| below the bottom {dev[-1]:.3e}", flush=True)
+        assert err <= FIELD_TOL
+        assert abs(cache.T_outlet[0] - hist[-1][0]) <= OUTLET_TOL
+    assert dev[-1] > 10 * dev[-4] and dev[-1] > 100.0
