@@ -117,9 +117,9 @@ def copy_rate(u, reps=5):
     return 2 * u.numel() * u.element_size() / best
 
 
-def cpu_baseline(w, threads, steps):
-    """The oracle (a restatement of the reference CPU path that keeps its
-    per-cell structure) on the host cores, on a bounded sample."""
+def _oracle_rate(w, threads, steps):
+    """cell-steps/s of the oracle on `steps` steps of workload w from its
+    initial field, with `threads` OpenMP threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import helpers as H
     O = entry.load_oracle()
@@ -128,10 +128,38 @@ def cpu_baseline(w, threads, steps):
     u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
     _, _, sec = orc.run(u0, 0.0, w["dt"], steps, 1.0 / 16.0)
     cells = len(w["gridx"]) * len(w["gridy"]) * len(w["gridz"])
-    return dict(value=cells * steps / sec, unit="cell-steps/s", cores=threads, kind="port",
-                sample=f"{steps} step(s) of the same 401x401x1002 workload, {sec:.1f} s wall, "
-                       f"OpenMP restatement in oracle/ (point-wise material lookups over all boreholes, "
-                       f"line-per-thread Thomas, quadratic turnaround loop as in the reference)")
+    return cells * steps / sec, sec
+
+
+def cpu_baseline(threads, c1_steps, c4_steps, single_thread):
+    """The oracle -- a restatement of the reference CPU path that keeps its
+    per-cell structure (point-wise material lookups over all boreholes,
+    line-per-thread Thomas, the quadratic turnaround loop) -- timed on the
+    host cores: the single borehole for its whole hour with `threads` threads
+    and with one, one step of the 3x3 array (the headline workload: `value`)
+    and one step of the 5x5 array.  Cost per step is constant over a run, so
+    the rates are cell-steps/s."""
+    from geothermalwells_jl_amd import workloads as W
+    runs = []
+
+    def run(name, w, thr, steps):
+        rate, sec = _oracle_rate(w, thr, steps)
+        runs.append({"workload": name, "threads": thr, "steps": steps, "seconds": sec, "cell_steps_per_s": rate})
+        return rate
+
+    c0 = W.c0()
+    run("single borehole 41x41x202, whole hour", c0, threads, c0["nsteps"])
+    if single_thread:
+        run("single borehole 41x41x202, whole hour", c0, 1, c0["nsteps"])
+    value = run("3x3 array 401x401x1002", W.c1(), threads, c1_steps)
+    if c4_steps > 0:
+        run("5x5 array 601x601x2002", W.c4(), threads, c4_steps)
+    return dict(value=value, unit="cell-steps/s", cores=threads, kind="port",
+                host_cpus=os.cpu_count(),
+                sample=f"{c1_steps} step(s) of the 401x401x1002 workload with {threads} threads (the host's CPU "
+                       f"share; os.cpu_count() = {os.cpu_count()}); OpenMP restatement of the reference CPU path "
+                       f"in oracle/; the other runs are listed in `runs`",
+                runs=runs)
 
 
 def _init_state(gw, cache, ws, dtype):
@@ -314,14 +342,104 @@ def bench_adjoint(gw, steps, seg, n_fd):
     return res
 
 
+def run_horizon(gw, name, precisions, dt, every, out_path):
+    """A workload's whole simulated span (the 3x3 array: 30 days; the 64
+    ragged scenarios: 7 days, batched; the 5x5 array: 1 year) in each
+    precision, the contexts stepped in alternating chunks of `every` steps
+    (default one simulated day).  At every chunk end: the outlet
+    temperatures and the field's range.  Prints one JSON line with the
+    largest and final outlet difference of FP32 against FP64 (when both ran)
+    and writes the series to out_path (.npz) when given."""
+    import torch
+    from geothermalwells_jl_amd import workloads as W
+    if name == "c2":
+        ws = W.c2()
+    else:
+        ws = [getattr(W, name)(dt=dt) if (name == "c4" and dt) else getattr(W, name)()]
+    dt = dt or ws[0]["dt"]
+    span = {"c1": 30 * 86400.0, "c2": 7 * 86400.0, "c4": 365 * 86400.0}[name]
+    nsteps = int(round(span / dt))
+    every = every or int(round(86400.0 / dt))
+    runs = {}
+    for prec in precisions:
+        dtype = torch.float64 if prec == "f64" else torch.float32
+        cache = (gw.create_batch_cache([W.scenario_kwargs(w) for w in ws], precision=prec) if len(ws) > 1
+                 else gw.create_cache(precision=prec, **W.scenario_kwargs(ws[0])))
+        runs[prec] = dict(cache=cache, u=_init_state(gw, cache, ws, dtype), T=[], lo=[], hi=[])
+    t_start = time.perf_counter()
+    done = 0
+    while done < nsteps:
+        n = min(every, nsteps - done)
+        for prec, r in runs.items():
+            c = r["cache"]
+            gw._abi.check(c._lib.gw_steps(c.ctx, c.check_state(r["u"]), done * dt, dt, n))
+        for prec, r in runs.items():
+            r["cache"].sync()
+            r["T"].append(r["cache"].T_outlet)
+            lo, hi = torch.aminmax(r["u"])
+            r["lo"].append(float(lo))
+            r["hi"].append(float(hi))
+        done += n
+        if done % (10 * every) == 0 or done == nsteps:
+            msg = " ".join(f"{p}: outlet {r['T'][-1].min():.6f}..{r['T'][-1].max():.6f} field "
+                           f"{r['lo'][-1]:.4f}..{r['hi'][-1]:.4f}" for p, r in runs.items())
+            print(f"horizon {name} step {done}/{nsteps} ({done * dt / 86400:.1f} d, "
+                  f"{time.perf_counter() - t_start:.0f} s): {msg}", file=sys.stderr, flush=True)
+    res = {"horizon": name, "dt_s": dt, "steps": nsteps, "days": nsteps * dt / 86400, "sample_every": every,
+           "seconds": time.perf_counter() - t_start}
+    for prec, r in runs.items():
+        T = np.array(r["T"])
+        res[prec] = {"outlet_final": [float(x) for x in T[-1]], "field_min": min(r["lo"]), "field_max": max(r["hi"]),
+                     "finite": bool(np.all(np.isfinite(T)))}
+    if "f64" in runs and "f32" in runs:
+        d = np.abs(np.array(runs["f32"]["T"]) - np.array(runs["f64"]["T"]))
+        res["outlet_err_fp32_vs_fp64_max_K"] = float(d.max())
+        res["outlet_err_fp32_vs_fp64_final_K"] = float(d[-1].max())
+    if out_path:
+        arrays = {"t": dt * every * np.arange(1, len(next(iter(runs.values()))["T"]) + 1)}
+        for prec, r in runs.items():
+            arrays[f"T_{prec}"] = np.array(r["T"])
+            arrays[f"min_{prec}"] = np.array(r["lo"])
+            arrays[f"max_{prec}"] = np.array(r["hi"])
+        np.savez(out_path, **arrays)
+    return res
+
+
+def compare_horizons(path_a, path_b):
+    """Outlet difference between the FP32 series of one horizon file and the
+    FP64 series of another (runs made in separate calls)."""
+    a, b = np.load(path_a, allow_pickle=False), np.load(path_b, allow_pickle=False)
+    Ta = a["T_f32"] if "T_f32" in a else a["T_f64"]
+    Tb = b["T_f64"] if "T_f64" in b else b["T_f32"]
+    if Ta.shape != Tb.shape or not np.array_equal(a["t"], b["t"]):
+        raise ValueError("the two series are not sampled at the same times")
+    d = np.abs(Ta - Tb)
+    return {"samples": int(len(a["t"])), "t_end_s": float(a["t"][-1]), "outlet_err_max_K": float(d.max()),
+            "outlet_err_final_K": float(d[-1].max()), "outlet_err_max_at_s": float(a["t"][int(d.max(axis=1).argmax())])}
+
+
+def _cpu_share():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when the host sets it (a
+    GPU host's share of its cores), else every core."""
+    n = os.cpu_count() or 1
+    try:
+        return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", n))))
+    except ValueError:
+        return n
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the 3x3 array on the CPU")
+    ap.add_argument("--cpu-c4-steps", type=int, default=1, help="steps of the 5x5 array on the CPU (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=_cpu_share(),
+                    help="OpenMP threads of the CPU baseline (default: the host's CPU share, OMP_NUM_THREADS "
+                         "when set, else os.cpu_count())")
+    ap.add_argument("--no-cpu-single-thread", action="store_true", help="skip the 1-thread CPU run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=30,
                     help="timed steps of the 64-scenario batched comparison (0: skip)")
@@ -338,7 +456,19 @@ def main():
                     help="layer conductivities checked by central differences (besides the inlet)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--horizon", choices=["c1", "c2", "c4"],
+                    help="instead of the benchmark: run the workload's whole simulated span (3x3 array, 64 "
+                         "scenarios batched, 5x5 array) and print its outlet errors")
+    ap.add_argument("--horizon-precision", default="f64,f32", help="precisions of --horizon (comma separated)")
+    ap.add_argument("--horizon-dt", type=float, default=0.0, help="dt of --horizon c4 (default: 600 s)")
+    ap.add_argument("--horizon-every", type=int, default=0, help="outlet sample interval in steps (default: a day)")
+    ap.add_argument("--horizon-out", default="", help="write the --horizon series to this .npz")
+    ap.add_argument("--horizon-compare", nargs=2, metavar=("F32_NPZ", "F64_NPZ"),
+                    help="compare two --horizon-out files (no GPU)")
     args = ap.parse_args()
+    if args.horizon_compare:
+        print(json.dumps(compare_horizons(*args.horizon_compare)), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step pipeline runs on a single device (a field split over "
               "GPUs would shard as z-slabs with a 3-level halo exchange per step, not implemented); run with --gpus 1",
@@ -354,6 +484,11 @@ def main():
         torch.cuda.synchronize()
         del a, b
     gw = entry.load_package()
+    if args.horizon:
+        res = run_horizon(gw, args.horizon, args.horizon_precision.split(","), args.horizon_dt,
+                          args.horizon_every, args.horizon_out)
+        print(json.dumps(res), flush=True)
+        return
     from geothermalwells_jl_amd import workloads as W
     w = W.c1()
     cells = W.cells(w)
@@ -448,7 +583,8 @@ def main():
         torch.cuda.empty_cache()
         out["adjoint"] = bench_adjoint(gw, args.adjoint_steps, args.adjoint_checkpoint, args.adjoint_fd)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_threads, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_threads, args.cpu_steps, args.cpu_c4_steps,
+                                           not args.no_cpu_single_thread)
     print(json.dumps(out), flush=True)
 
 
