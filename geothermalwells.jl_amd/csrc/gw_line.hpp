@@ -100,15 +100,18 @@ GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
 
 // Waves per SIMD the register budget aims at: a tile of NW waves puts NW/4 on
 // each SIMD, so four-wave tiles run three to a CU when the line fits in 168
-// registers (S <= 52), eight-wave tiles two to a CU when it fits in 128.
-template <class Cfg, int MODE>
+// registers (S <= 52), eight-wave tiles two to a CU when it fits in 128.  An
+// FP32 line takes half the registers: four-wave FP32 tiles aim at five per CU
+// (96 registers) up to S = 32 and four (128) up to S = 48.
+template <class Cfg, class TS, int MODE>
 constexpr int line_min_waves() {
+  if (sizeof(TS) == 4 && Cfg::NW <= 4 && Cfg::S > 32 && Cfg::S <= 40) return 4;
   if (MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_ZY) return 2;
   return Cfg::NW <= 4 ? (Cfg::S <= 52 ? 3 : 2) : (Cfg::S <= 32 ? 4 : 2);
 }
 
 template <class Cfg, class TS, int MODE>
-__global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k_line(DevView v, int dir, TS* out, const TS* in,
+__global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) void k_line(DevView v, int dir, TS* out, const TS* in,
                                                           double c, double a1, double a2, double a3,
                                                           int64_t tile0, int64_t ntiles) {
   using MT = ModeTraits<MODE>;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, MODE>())) void k
   // of the next chunk are fetched while the current one computes.  The long
   // FP64 segments of the fused z-step modes use chunks of two (their
   // registers are otherwise short at three waves per SIMD).
-  constexpr int KC = (MT::halo && S > 40 && sizeof(TS) == 8) ? 2 : 4;
+  constexpr int KC = ((MT::halo && S > 40 && sizeof(TS) == 8) || (sizeof(TS) == 4 && Cfg::NW <= 4 && S > 32 && S <= 40)) ? 2 : 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
   if (MT::impl) {
     // forward: y_l = b_l - fw_l y_{l-1}
