@@ -118,12 +118,14 @@ def c2(n_scen=64):
         T_in = rng.uniform(5.0, 30.0)
         Q = rng.uniform(0.005, 0.015)
         bh = coaxial(0.0, 0.0, h=h, Q=Q)
-        gx, _ = _grid_with_n(n_xy, bh)
+        gx, gp = _grid_with_n(n_xy, bh)
         zmax = round(1.05 * h, 1)
         dz = round(zmax / nz_cells, 3)
         gz = create_uniform_gridz_with_borehole_depths(zmin=0.0, zmax=zmax, dz=dz, boreholes=(bh,))
         out.append(dict(name=f"c2_{s}", gridx=gx, gridy=gx.copy(), gridz=gz, materials=homogeneous(k_rock=k_rock),
-                        boreholes=(bh,), inlet_model=ConstantInlet(T_in), dt=120.0, nsteps=5040, ic=_ic()))
+                        boreholes=(bh,), inlet_model=ConstantInlet(T_in), dt=120.0, nsteps=5040, ic=_ic(),
+                        draws=dict(n_xy=n_xy, nz_cells=nz_cells, h=h, k_rock=k_rock, T_in=T_in, Q=Q, zmax=zmax, dz=dz,
+                                   **gp)))
     return out
 
 
@@ -170,3 +172,27 @@ def scenario_kwargs(w):
 
 def cells(w):
     return len(w["gridx"]) * len(w["gridy"]) * len(w["gridz"])
+
+
+def _grid_summary(w):
+    return {a: {"n": int(len(w[a])), "first": float(w[a][0]), "last": float(w[a][-1]),
+                "sum": float(np.sum(w[a])), "min_step": float(np.min(np.diff(w[a])))}
+            for a in ("gridx", "gridy", "gridz")}
+
+
+def manifest():
+    """The drawn values and grid summaries of every configuration (the
+    committed workload_manifest.json holds them, so a run on any machine can
+    check it regenerated the same inputs)."""
+    out = {"rng": "SplitMix64, u = (x >> 11) * 2^-53", "configurations": {}}
+    for name, fn in (("c0", c0), ("c1", c1), ("c3", c3), ("c4", c4)):
+        w = fn()
+        e = {"dt_s": w["dt"], "steps": w["nsteps"], "grids": _grid_summary(w),
+             "boreholes": [list(b.args) for b in w["boreholes"]]}
+        inl = w["inlet_model"]
+        e["inlet"] = list(inl.values) if isinstance(inl, PerWellInlet) else float(inl.T_const)
+        if isinstance(w["materials"], StratifiedMaterialProperties):
+            e["k_layers"] = [float(k) for k in w["materials"].k_rock_layers]
+        out["configurations"][name] = e
+    out["configurations"]["c2"] = [dict(w["draws"], grids=_grid_summary(w)) for w in c2()]
+    return out
