@@ -13,6 +13,8 @@ from .grids import compute_domain, create_adaptive_grid_1d, create_uniform_gridz
 from .physics import (AbstractBorehole, AbstractMaterialProperties, Borehole, ConstantInlet, CustomInlet,
                       HeatExchangerInlet, HomogenousMaterialProperties, PerWellInlet, StratifiedMaterialProperties,
                       initial_condition_thermal_gradient)
+from .util import (data_brown_single_well_b, data_brown_single_well_c, data_hu, data_li, extract_x_profile,
+                   get_temperatures_along_z_single_well)
 from .solver import (ADI_and_ADV_callback_, ODEProblem, ROCK2, eigen_estimator, rhs_diffusion_z_,
                      rock2_coefficients, solve, step_plan)
 

@@ -80,6 +80,11 @@ SIGNATURES = {
     "gw_adaptive_grid_1d": (ctypes.c_int64, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, ctypes.POINTER(GwBorehole), ctypes.c_int32,
                                              ctypes.c_int32, c_double_p, ctypes.c_int64]),
+    "gw_pipe_level_means": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, c_double_p, c_double_p,
+                                           ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "gw_extract_x_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, c_double_p, c_double_p, ctypes.c_double, ctypes.c_double,
+                                            c_double_p, ctypes.c_void_p]),
     "gw_state_elems": (ctypes.c_int64, [ctypes.c_void_p]),
     "gw_scenario_offset": (ctypes.c_int64, [ctypes.c_void_p, ctypes.c_int32]),
     "gw_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),

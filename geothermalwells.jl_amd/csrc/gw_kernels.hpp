@@ -204,4 +204,13 @@ hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipSt
 // Semi-Lagrangian update of every pipe column, in place.
 hipError_t launch_pipe_advect(const DevView& v, int prec, void* phi, hipStream_t st);
 
+// Post-processing (src/util.jl:274-356).  Per-level means of phi over two
+// lists of column offsets (off[0 .. n_in) then off[n_in .. n_in + n_an)), for
+// levels 0 .. nlev-1: means[0 .. nlev) and means[nlev .. 2 nlev).
+hipError_t launch_level_means(int prec, const void* phi, const int64_t* off, int n_in, int n_an, int nlev,
+                              double* means, hipStream_t st);
+// out[i] = phi(k_lo, j, i) (k_lo == k_hi) or w_lo phi(k_lo, j, i) + w_hi phi(k_hi, j, i), i < nx.
+hipError_t launch_x_profile(int prec, const void* phi, int nx, int ny, int nz, int j, int k_lo, int k_hi,
+                            double w_lo, double w_hi, double* out, hipStream_t st);
+
 }  // namespace gw

@@ -225,6 +225,23 @@ int64_t gw_uniform_gridz(double zmin, double zmax, double dz, const gw_borehole 
 int64_t gw_adaptive_grid_1d(double xmin, double xmax, double dx_fine, double growth_factor, double dx_max,
                             const gw_borehole *bh, int32_t n_bh, int32_t direction, double *out, int64_t cap);
 
+/* get_temperatures_along_z_single_well(phi, cache) (src/util.jl:330-356) on
+ * the resident field: for the levels with gridz < h of the scenario's first
+ * borehole (their count in *nlev), the mean over the Idx_list_Inner columns
+ * (T_inner) and over the Idx_list_Outer columns (T_outer), FP64, host arrays
+ * of at least that count (cap).  T_inner = T_outer = NULL only reports *nlev.
+ * Synchronous. */
+int gw_pipe_level_means(gw_ctx *ctx, const void *phi, int32_t scen, double *T_inner, double *T_outer, int32_t cap,
+                        int32_t *nlev);
+/* extract_x_profile(T, gridx, gridy, gridz, z_target, xc, yc, full_profile=true)
+ * (src/util.jl:274-318) of a device field T (nx, ny, nz as everywhere here):
+ * the x-line at the y index closest to yc, linearly interpolated between the
+ * levels bracketing z_target (the first or last level outside the grid),
+ * into the host array profile[nx].  No context; runs on `stream` (NULL: the
+ * null stream), synchronous. */
+int gw_extract_x_profile(const void *T, int32_t precision, int32_t nx, int32_t ny, int32_t nz, const double *gridy,
+                         const double *gridz, double z_target, double yc, double *profile, void *stream);
+
 /* Introspection (cache fields the reference exposes). */
 int64_t gw_state_elems(gw_ctx *ctx);
 int64_t gw_scenario_offset(gw_ctx *ctx, int32_t scen);
