@@ -209,6 +209,7 @@ struct CTab {
   RedChunk* d_agroup = nullptr;
   double* d_partial = nullptr;
   std::vector<Dep> h_dep;           // host copy of the departure table (adjoint)
+  std::string adv_err;              // why an advection sub-step of this length would yield NaN (none: empty)
   int nmeans = 0;
   int npartial = 0;                 // doubles of d_partial
   int nrchunk = 0;
@@ -274,6 +275,10 @@ struct gw_ctx {
   void* buf2 = nullptr;
   // device the context was created on, and that device's copy of the
   // file-scope scratch array (gw_adi_adv_noalloc)
+  // post-processing: per scenario, the device column offsets of the inner
+  // then the annulus index list (built on first use), and an output buffer
+  std::map<int, int64_t*> post_off;
+  double* post_buf = nullptr;
   int device = 0;
   double* scratch_dev = nullptr;
   std::string scratch_err;  // why scratch_dev is NULL
@@ -288,6 +293,8 @@ struct gw_ctx {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph_exec2) (void)hipGraphExecDestroy(graph_exec2);
     if (buf2) (void)hipFree(buf2);
+    for (auto& kv : post_off) (void)hipFree(kv.second);
+    if (post_buf) (void)hipFree(post_buf);
     for (auto& kv : ctabs) {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
@@ -604,15 +611,18 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       B.col_an0 = ctx.bh_col_an0[gb]; B.n_an = ctx.bh_n_an[gb];
       B.inlet_kind = s.inlet.kind;
       B.inlet_value = s.inlet.value;
+      // Configurations for which the reference's advection yields NaN: the
+      // sub-step refuses them (the diffusion operators of this c still run)
+      auto adv_fail = [&](const std::string& msg) { if (t.adv_err.empty()) t.adv_err = msg; };
       if (B.n_in == 0)
-        return fail(GW_EINVAL, "borehole " + std::to_string(b + 1) + " of scenario " + std::to_string(si) +
-                                   " has no inner-pipe grid columns: its outlet mean would be 0/0 (src/solver.jl:257)");
+        adv_fail("borehole " + std::to_string(b + 1) + " of scenario " + std::to_string(si) +
+                 " has no inner-pipe grid columns: its outlet mean would be 0/0 (src/solver.jl:257)");
       const double zmax = bh.v_inner * c;
       int nout = 0;
       for (int k = 0; k < s.nz; ++k) if (s.gz[k] <= zmax) ++nout;
       // gz is sorted, so the outlet levels are k < nout
       B.n_outlev = nout;
-      if (nout == 0) return fail(GW_EINVAL, "no outlet levels (gridz[1] > v_inner*dt)");
+      if (nout == 0) adv_fail("no outlet levels (gridz[1] > v_inner*dt)");
       B.dep_off = (int)dep.size();
       int klo = 1 << 30, khi = -1;
       for (int k = 0; k < s.countz; ++k) {
@@ -634,17 +644,23 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
             interp(s.gz, zd, di.kl, di.kr, di.a);
             di.type = DEP_OWN;
           }
-          if (di.kl == di.kr || !std::isfinite(di.a))
-            return fail(GW_EINVAL, "degenerate inner-pipe departure point (zero flow rate or a turnaround "
-                                   "departure above the surface): the reference yields NaN (src/solver.jl:310,373)");
+          if (di.kl == di.kr || !std::isfinite(di.a)) {
+            adv_fail("degenerate inner-pipe departure point (zero flow rate or a turnaround departure above the "
+                     "surface): the reference yields NaN (src/solver.jl:310,373)");
+            di = Dep{};
+            di.type = DEP_KEEP;
+          }
           const double za = std::fma(-bh.v_outer, c, z);  // :348
           if (za <= 0.0) {
             da.type = DEP_INLET;
           } else {
             interp(s.gz, za, da.kl, da.kr, da.a);
             da.type = DEP_OWN;
-            if (da.kl == da.kr || !std::isfinite(da.a))
-              return fail(GW_EINVAL, "degenerate annulus departure point (src/solver.jl:352)");
+            if (da.kl == da.kr || !std::isfinite(da.a)) {
+              adv_fail("degenerate annulus departure point (src/solver.jl:352)");
+              da = Dep{};
+              da.type = DEP_KEEP;
+            }
           }
         }
         dep.push_back(di);
@@ -652,8 +668,8 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       }
       if (khi >= 0) {
         if (B.n_an == 0)
-          return fail(GW_EINVAL, "borehole " + std::to_string(b + 1) +
-                                     " has no annulus grid columns: the turnaround mean would be 0/0 (src/solver.jl:334)");
+          adv_fail("borehole " + std::to_string(b + 1) +
+                   " has no annulus grid columns: the turnaround mean would be 0/0 (src/solver.jl:334)");
         B.klo = klo;
         B.nlev = khi - klo + 1;
       } else {
@@ -734,6 +750,7 @@ double stability_bound(double c3) {
 // tables' own buffers.
 int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* jacc = nullptr,
                     double* scratch_mem = nullptr) {
+  if (!t->adv_err.empty()) return fail(GW_EINVAL, t->adv_err);
   DevView v = ctx.view(t);
   if (scratch_mem) {
     v.partial = scratch_mem;
@@ -1230,6 +1247,7 @@ int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
   CTab* tab = nullptr;
   int rc = prepare_steps(*ctx, phi, dt, tab);
   if (rc) return rc;
+  if (!tab->adv_err.empty()) return fail(GW_EINVAL, tab->adv_err);
   void* B = ctx->buf2;
   rc = fused_zy(*ctx, tab, phi, B, dt);
   if (rc) return rc;
@@ -1357,6 +1375,77 @@ double gw_eigen_estimate(gw_ctx* ctx, int32_t s) {
   return ctx->sc[s].eigen;
 }
 
+int gw_pipe_level_means(gw_ctx* ctx, const void* phi, int32_t s, double* T_inner, double* T_outer, int32_t cap,
+                        int32_t* nlev_out) {
+  if (!ctx || s < 0 || s >= (int)ctx->sc.size()) return fail(GW_EINVAL, "bad scenario");
+  const Scen& sc = ctx->sc[s];
+  int nlev = 0;  // Nz_till_h = sum(gridz .< h) of the scenario's first borehole
+  for (double z : sc.gz) if (z < sc.bh[0].h) ++nlev;
+  if (nlev_out) *nlev_out = nlev;
+  if (!T_inner && !T_outer) return GW_OK;
+  if (!phi || !T_inner || !T_outer || cap < nlev) return fail(GW_EINVAL, "null argument or output shorter than the level count");
+  const int n_in = (int)sc.idx_inner.size(), n_an = (int)sc.idx_outer.size();
+  if (n_in == 0 || n_an == 0) return fail(GW_EINVAL, "empty pipe index list: its mean is undefined");
+  try {
+    auto it = ctx->post_off.find(s);
+    if (it == ctx->post_off.end()) {
+      std::vector<int64_t> off;
+      for (auto& e : sc.idx_inner) off.push_back(sc.off + ((int64_t)e[1] + (int64_t)sc.ny * e[0]) * sc.nz);
+      for (auto& e : sc.idx_outer) off.push_back(sc.off + ((int64_t)e[1] + (int64_t)sc.ny * e[0]) * sc.nz);
+      it = ctx->post_off.emplace(s, dev_upload(off)).first;
+    }
+    if (!ctx->post_buf) hip_check(hipMalloc(&ctx->post_buf, sizeof(double) * 2 * std::max(ctx->nz_max, ctx->nx_max)));
+  } catch (HipFail& e) {
+    return fail(GW_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e.e));
+  }
+  int rc = hip_status(launch_level_means(ctx->prec, phi, ctx->post_off[s], n_in, n_an, nlev, ctx->post_buf,
+                                         ctx->stream), "level means");
+  if (rc) return rc;
+  if ((rc = hip_status(hipMemcpyAsync(T_inner, ctx->post_buf, sizeof(double) * nlev, hipMemcpyDeviceToHost,
+                                      ctx->stream), "copy")))
+    return rc;
+  if ((rc = hip_status(hipMemcpyAsync(T_outer, ctx->post_buf + nlev, sizeof(double) * nlev, hipMemcpyDeviceToHost,
+                                      ctx->stream), "copy")))
+    return rc;
+  return hip_status(hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int gw_extract_x_profile(const void* T, int32_t precision, int32_t nx, int32_t ny, int32_t nz, const double* gridy,
+                         const double* gridz, double z_target, double yc, double* profile, void* stream) {
+  if (!T || !gridy || !gridz || !profile || nx <= 0 || ny <= 0 || nz <= 0 ||
+      (precision != GW_F64 && precision != GW_F32_STATE))
+    return fail(GW_EINVAL, "bad argument");
+  // y index closest to yc (the first of equals), src/util.jl:276
+  int j = 0;
+  for (int q = 1; q < ny; ++q)
+    if (std::fabs(gridy[q] - yc) < std::fabs(gridy[j] - yc)) j = q;
+  // bracketing levels and weights, src/util.jl:279-303
+  int k_lo = 0, k_hi = 0;
+  double w_lo = 1.0, w_hi = 0.0;
+  if (z_target <= gridz[0]) {
+    k_lo = k_hi = 0;
+  } else if (z_target >= gridz[nz - 1]) {
+    k_lo = k_hi = nz - 1;
+  } else {
+    k_lo = 0;
+    for (int q = 0; q < nz; ++q) if (gridz[q] <= z_target) k_lo = q;    // findlast(gridz .<= z)
+    k_hi = nz - 1;
+    for (int q = nz - 1; q >= 0; --q) if (gridz[q] >= z_target) k_hi = q;  // findfirst(gridz .>= z)
+    if (k_lo != k_hi) {
+      w_hi = (z_target - gridz[k_lo]) / (gridz[k_hi] - gridz[k_lo]);
+      w_lo = 1.0 - w_hi;
+    }
+  }
+  double* d = nullptr;
+  if (hipMalloc(&d, sizeof(double) * nx) != hipSuccess) return fail(GW_ENOMEM, "device allocation failed");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  int rc = hip_status(launch_x_profile(precision, T, nx, ny, nz, j, k_lo, k_hi, w_lo, w_hi, d, st), "x profile");
+  if (!rc) rc = hip_status(hipMemcpyAsync(profile, d, sizeof(double) * nx, hipMemcpyDeviceToHost, st), "copy");
+  if (!rc) rc = hip_status(hipStreamSynchronize(st), "sync");
+  (void)hipFree(d);
+  return rc;
+}
+
 int gw_sync(gw_ctx* ctx) {
   if (!ctx) return fail(GW_EINVAL, "null argument");
   return hip_status(hipStreamSynchronize(ctx->stream), "sync");
@@ -1373,6 +1462,7 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
   CTab* tab = nullptr;
   int rc = prepare_steps(*ctx, phi, dt, tab);
   if (rc) return rc;
+  if (!tab->adv_err.empty()) return fail(GW_EINVAL, tab->adv_err);
   hipEvent_t ev[5] = {};
   for (auto& e : ev)
     if ((rc = hip_status(hipEventCreate(&e), "event create"))) return rc;
@@ -1697,6 +1787,7 @@ int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void
     const size_t bytes = (size_t)C.cells_total * sizeof(double);
     switch (op) {
       case 0: {
+        if (!tab->adv_err.empty()) return fail(GW_EINVAL, tab->adv_err);
         const AdjPipe ap = build_adj_pipe(C, *tab, db);
         rc = hip_status(hipMemcpyAsync(o, i, bytes, hipMemcpyDeviceToDevice, C.stream), "copy");
         if (!rc) rc = hip_status(launch_adj_pipe(v, ag, ap, o, nullptr, C.stream), "adjoint advection");
@@ -1754,6 +1845,7 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     const double c = dt / 2;
     const size_t cells = (size_t)C.cells_total, bytes = cells * sizeof(double);
     const int nbh = C.nbh_total;
+    if (!tab->adv_err.empty()) return fail(GW_EINVAL, tab->adv_err);
     DevBufs db;
     const AdjGeo ag = make_adj_geo(C, db);
     const AdjPipe ap = build_adj_pipe(C, *tab, db);
