@@ -91,7 +91,7 @@ class Cache:
     advection sub-step."""
 
     def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                 line_window=None, line_two_tiles=False, line_halo_whole=False):
+                 line_window=None, line_two_tiles=False, line_halo_whole=False, line_concurrent=False):
         self._scen = scenarios
         self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
         if self.backend.type != "cuda":
@@ -110,11 +110,12 @@ class Cache:
         prob.rock2_c3 = self.rock2_c3
         if line_window not in (None, 64, 32):
             raise ValueError("line_window is None (autotune), 64 or 32")
-        if (line_two_tiles or line_halo_whole) and line_window is None:
-            raise ValueError("line_two_tiles / line_halo_whole go with a forced line_window")
+        if (line_two_tiles or line_halo_whole or line_concurrent) and line_window is None:
+            raise ValueError("line_two_tiles / line_halo_whole / line_concurrent go with a forced line_window")
         prob.flags = ({None: 0, 64: _abi.GW_FLAG_LINE_W64, 32: _abi.GW_FLAG_LINE_W32}[line_window] |
                       (_abi.GW_FLAG_LINE_TWO_TILES if line_two_tiles else 0) |
-                      (_abi.GW_FLAG_HALO_WHOLE if line_halo_whole else 0))
+                      (_abi.GW_FLAG_HALO_WHOLE if line_halo_whole else 0) |
+                      (_abi.GW_FLAG_LINE_CONCURRENT if line_concurrent else 0))
         h = ctypes.c_void_p()
         _abi.check(lib.gw_create(ctypes.byref(prob), ctypes.byref(h)))
         self._ctx = h
@@ -219,14 +220,14 @@ def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inl
 
 
 def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                       line_window=None):
+                       line_window=None, line_concurrent=False):
     """Many independent scenarios (each a dict with the create_cache keywords
     gridx, gridy, gridz, materials, boreholes, inlet_model) advanced together:
     every kernel launch covers all of them."""
     sc = [_Scenario(d["gridx"], d["gridy"], d["gridz"], d["materials"], d["boreholes"], d["inlet_model"])
           for d in scenarios]
     return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3,
-                 line_window=line_window)
+                 line_window=line_window, line_concurrent=line_concurrent)
 
 
 # ---------------------------------------------------------------------------

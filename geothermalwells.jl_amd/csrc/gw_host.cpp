@@ -279,6 +279,10 @@ struct gw_ctx {
   // then the annulus index list (built on first use), and an output buffer
   std::map<int, int64_t*> post_off;
   double* post_buf = nullptr;
+  // side streams of the concurrent class launches (batched contexts)
+  LineStreams ls{};
+  bool has_ls = false;
+  const LineStreams* lsp() const { return has_ls ? &ls : nullptr; }
   int device = 0;
   double* scratch_dev = nullptr;
   std::string scratch_err;  // why scratch_dev is NULL
@@ -302,6 +306,13 @@ struct gw_ctx {
       (void)hipFree(kv.second.d_agroup);
     }
     for (void* p : allocs) (void)hipFree(p);
+    if (has_ls) {
+      for (int q = 0; q < kSideStreams; ++q) {
+        (void)hipStreamDestroy(ls.side[q]);
+        (void)hipEventDestroy(ls.join[q]);
+      }
+      (void)hipEventDestroy(ls.fork);
+    }
     if (own_stream && stream) (void)hipStreamDestroy(stream);
   }
   DevView view(const CTab* t) const {
@@ -790,17 +801,17 @@ int adi_adv_tab(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, do
   const double c = dt / 2;
   DevView v = ctx.view(tab);
   // explicit y  (src/solver.jl:444)
-  int rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "explicit y");
+  int rc = hip_status(launch_line(v, ctx.prec, LINE_EXPLICIT, 1, ctx.lw(LINE_EXPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "explicit y");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, phi, t, nullptr, scratch_mem);  // :447
   if (rc) return rc;
   // implicit x then explicit x  (:450, :455)
-  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), phi, phi, c, 0.0, 0.0, ctx.stream), "implicit+explicit x");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), phi, phi, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit+explicit x");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, phi, t + dt / 2, nullptr, scratch_mem);  // :458
   if (rc) return rc;
   // implicit y  (:461)
-  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream), "implicit y");
+  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
 }
 
 int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
@@ -850,7 +861,7 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
   if (p->n_scen <= 0 || !p->scen) return fail(GW_EINVAL, "no scenarios");
   if (p->precision != GW_F64 && p->precision != GW_F32_STATE) return fail(GW_EINVAL, "unknown precision");
   const int32_t width = p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32);
-  const int32_t variant = p->flags & (GW_FLAG_LINE_TWO_TILES | GW_FLAG_HALO_WHOLE);
+  const int32_t variant = p->flags & (GW_FLAG_LINE_TWO_TILES | GW_FLAG_HALO_WHOLE | GW_FLAG_LINE_CONCURRENT);
   if ((p->flags & ~(width | variant)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) || (variant && !width))
     return fail(GW_EINVAL, "unknown or conflicting flags (the launch variants need a forced width)");
   std::unique_ptr<gw_ctx> ctx(new gw_ctx());
@@ -859,6 +870,7 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
   if (ctx->force_lw) {
     if (variant & GW_FLAG_LINE_TWO_TILES) ctx->force_lw |= kLwTwoTiles;
     if (variant & GW_FLAG_HALO_WHOLE) ctx->force_lw |= kLwHaloWhole;
+    if (variant & GW_FLAG_LINE_CONCURRENT) ctx->force_lw |= kLwConcurrent;
   }
   ctx->c3 = p->rock2_c3 > 0 ? p->rock2_c3 : 1.0 / 16.0;
   ctx->zstab = stability_bound(ctx->c3);
@@ -936,6 +948,14 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     } else {
       hip_check(hipStreamCreate(&ctx->stream));
       ctx->own_stream = true;
+    }
+    if (p->n_scen > 1) {  // ragged batches launch several line classes per operator
+      hip_check(hipEventCreateWithFlags(&ctx->ls.fork, hipEventDisableTiming));
+      for (int q = 0; q < kSideStreams; ++q) {
+        hip_check(hipStreamCreateWithFlags(&ctx->ls.side[q], hipStreamNonBlocking));
+        hip_check(hipEventCreateWithFlags(&ctx->ls.join[q], hipEventDisableTiming));
+      }
+      ctx->has_ls = true;
     }
     for (auto& s : ctx->sc) {
       ctx->nx_max = std::max(ctx->nx_max, s.nx);
@@ -1126,7 +1146,7 @@ int ensure_buf2(gw_ctx& ctx) {
 }
 
 int fused_zy(gw_ctx& ctx, const CTab* tab, const void* in, void* out, double dt) {
-  return hip_status(launch_line(ctx.view(tab), ctx.prec, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), out, in, dt / 2, dt, ctx.c3, ctx.stream),
+  return hip_status(launch_line(ctx.view(tab), ctx.prec, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), out, in, dt / 2, dt, ctx.c3, ctx.stream, ctx.lsp()),
                     "z-step + explicit y");
 }
 
@@ -1138,13 +1158,13 @@ int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end
   DevView v = ctx.view(tab);
   int rc = sub_step_advect(ctx, tab, cur, t_end);
   if (rc) return rc;
-  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx.stream), "x");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "x");
   if (rc) return rc;
   rc = sub_step_advect(ctx, tab, cur, t_end + c, jacc);
   if (rc) return rc;
   if (last)
-    return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream), "implicit y");
-  return hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, ctx.lw(LINE_YZY, 1), next, cur, c, dt, ctx.c3, ctx.stream),
+    return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
+  return hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, ctx.lw(LINE_YZY, 1), next, cur, c, dt, ctx.c3, ctx.stream, ctx.lsp()),
                     "implicit y + z-step + explicit y");
 }
 
@@ -1165,10 +1185,14 @@ int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
     float best = 0.f;
     int best_lw = ctx.lw_sel[mode][dir];
     const bool halo = mode == LINE_ZY || mode == LINE_YZY;
-    for (int var = 0; var < (halo ? 8 : 4); ++var) {
-      const int lw = (var & 1 ? 32 : 64) | (var & 2 ? kLwTwoTiles : 0) | (var & 4 ? kLwHaloWhole : 0);
+    const int nvar = (halo ? 8 : 4) * (ctx.has_ls ? 2 : 1);
+    for (int var0 = 0; var0 < nvar; ++var0) {
+      // bit 3 of the variant: concurrent class launches (batched contexts)
+      const int var = (var0 & 3) | (halo ? (var0 & 4) : 0) | ((var0 >> (halo ? 3 : 2)) ? 8 : 0);
+      const int lw = (var & 1 ? 32 : 64) | (var & 2 ? kLwTwoTiles : 0) | (var & 4 ? kLwHaloWhole : 0) |
+                     (var & 8 ? kLwConcurrent : 0);
       auto run = [&]() {
-        return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream),
+        return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream, ctx.lsp()),
                           "line tune");
       };
       if ((rc = run())) break;  // first launch: code object load
@@ -1287,7 +1311,7 @@ int gw_line_op(gw_ctx* ctx, void* out, const void* in, double c, double dt_z, in
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, ctx->lw(mode, dir), out, in, c, dt_z, ctx->c3, ctx->stream),
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, ctx->lw(mode, dir), out, in, c, dt_z, ctx->c3, ctx->stream, ctx->lsp()),
                     "line operator");
 }
 
@@ -1324,7 +1348,7 @@ int gw_explicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, ctx->lw(LINE_EXPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream), "explicit");
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, ctx->lw(LINE_EXPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "explicit");
 }
 
 int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
@@ -1332,7 +1356,7 @@ int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, ctx->lw(LINE_IMPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream), "implicit");
+  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, ctx->lw(LINE_IMPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "implicit");
 }
 
 int gw_advect(gw_ctx* ctx, void* phi, double c, double t) {
@@ -1485,12 +1509,12 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
       void* cur = (n & 1) ? phi : B;
       void* next = last ? phi : ((n & 1) ? B : phi);
       if ((rc = rec(0)) || (rc = sub_step_advect(*ctx, tab, cur, te)) || (rc = rec(1))) break;
-      if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, ctx->lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx->stream),
+      if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, ctx->lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx->stream, ctx->lsp()),
                            "x")) || (rc = rec(2)))
         break;
       if ((rc = sub_step_advect(*ctx, tab, cur, te + c)) || (rc = rec(3))) break;
-      rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, ctx->lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx->stream), "y")
-                : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, ctx->lw(LINE_YZY, 1), next, cur, c, dt, ctx->c3, ctx->stream), "yzy");
+      rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, ctx->lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "y")
+                : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, ctx->lw(LINE_YZY, 1), next, cur, c, dt, ctx->c3, ctx->stream, ctx->lsp()), "yzy");
       if (rc || (rc = rec(4))) break;
       if ((rc = hip_status(hipEventSynchronize(ev[4]), "event sync"))) break;
       if ((rc = acc(0, 1, 1)) || (rc = acc(1, 2, 2)) || (rc = acc(2, 3, 3)) || (rc = acc(3, 4, last ? 5 : 4))) break;
@@ -1696,7 +1720,7 @@ int adj_recompute(gw_ctx& ctx, const CTab* tab, double* const* U, double* const*
                   double t_first_end, double dt) {
   const double c = dt / 2;
   const DevView v = ctx.view(tab);
-  int rc = hip_status(launch_line(v, GW_F64, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), wa, U[0], c, dt, ctx.c3, ctx.stream),
+  int rc = hip_status(launch_line(v, GW_F64, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), wa, U[0], c, dt, ctx.c3, ctx.stream, ctx.lsp()),
                       "z-step + explicit y");
   for (int m = 0; m < n && !rc; ++m) {
     const double t_end = t_first_end + (double)m * dt;
@@ -1704,18 +1728,18 @@ int adj_recompute(gw_ctx& ctx, const CTab* tab, double* const* U, double* const*
     DevView vk = v;
     vk.aux = E[m];
     if (!rc) rc = hip_status(launch_line(vk, GW_F64, LINE_IMPLICIT_EXPLICIT_KEEP, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0),
-                                         wa, wa, c, 0.0, 0.0, ctx.stream), "implicit + explicit x");
+                                         wa, wa, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit + explicit x");
     if (!rc) rc = sub_step_advect(ctx, tab, wa, t_end + c);
     if (rc) break;
     if (m + 1 < n) {
       DevView vy = v;
       vy.aux = U[m + 1];
       rc = hip_status(launch_line(vy, GW_F64, LINE_YZY_KEEP, 1, ctx.lw(LINE_YZY, 1), wb, wa, c, dt, ctx.c3,
-                                  ctx.stream), "implicit y + z-step + explicit y");
+                                  ctx.stream, ctx.lsp()), "implicit y + z-step + explicit y");
       std::swap(wa, wb);
     } else {
       rc = hip_status(launch_line(v, GW_F64, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), U[m + 1], wa, c, 0.0, 0.0,
-                                  ctx.stream), "implicit y");
+                                  ctx.stream, ctx.lsp()), "implicit y");
     }
   }
   return rc;
@@ -1806,7 +1830,7 @@ int gw_adjoint_op(gw_ctx* ctx, int32_t op, void* out, const void* in, const void
       }
       case 3:
         rc = hip_status(launch_line(v, GW_F64, LINE_EXPLICIT, 1, C.lw(LINE_EXPLICIT, 1), o, i, c, 0.0, 0.0,
-                                    C.stream), "explicit y");
+                                    C.stream, C.lsp()), "explicit y");
         break;
       case 4:
       case 6:

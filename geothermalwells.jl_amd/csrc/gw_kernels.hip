@@ -335,20 +335,19 @@ hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double
 int line_window_default(int nmax) { return nmax <= kLineClassMax[8] ? 64 : 32; }
 
 hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, void* out, const void* in,
-                       double c, double dt_z, double c3, hipStream_t st) {
+                       double c, double dt_z, double c3, hipStream_t st, const LineStreams* ls) {
   const double a1 = 2.0 * c3 * dt_z, a2 = 0.5 * dt_z, a3 = dt_z;
-  if (lw & kLwFlags) {
-    DevView v2 = v;
-    if (lw & kLwTwoTiles) v2.lds_floor = kLdsTwoTiles;
-    if (lw & kLwHaloWhole) v2.halo_whole = 1;
-    return launch_line(v2, prec, mode, dir, lw & ~kLwFlags, out, in, c, dt_z, c3, st);
-  }
-  if (lw == 64)
-    return prec == 0 ? line_w64_f64(v, mode, dir, out, in, c, a1, a2, a3, st)
-                     : line_w64_f32(v, mode, dir, out, in, c, a1, a2, a3, st);
-  if (lw == 32)
-    return prec == 0 ? line_w32_f64(v, mode, dir, out, in, c, a1, a2, a3, st)
-                     : line_w32_f32(v, mode, dir, out, in, c, a1, a2, a3, st);
+  DevView v2 = v;
+  if (lw & kLwTwoTiles) v2.lds_floor = kLdsTwoTiles;
+  if (lw & kLwHaloWhole) v2.halo_whole = 1;
+  const LineStreams* side = (lw & kLwConcurrent) ? ls : nullptr;
+  const int w = lw & ~kLwFlags;
+  if (w == 64)
+    return prec == 0 ? line_w64_f64(v2, mode, dir, out, in, c, a1, a2, a3, st, side)
+                     : line_w64_f32(v2, mode, dir, out, in, c, a1, a2, a3, st, side);
+  if (w == 32)
+    return prec == 0 ? line_w32_f64(v2, mode, dir, out, in, c, a1, a2, a3, st, side)
+                     : line_w32_f32(v2, mode, dir, out, in, c, a1, a2, a3, st, side);
   return hipErrorInvalidValue;
 }
 

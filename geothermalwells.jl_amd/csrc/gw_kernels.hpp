@@ -76,7 +76,19 @@ struct DevView {
 // autotune decides per operator.
 constexpr int kLwTwoTiles = 1;
 constexpr int kLwHaloWhole = 2;
-constexpr int kLwFlags = kLwTwoTiles | kLwHaloWhole;
+// kLwConcurrent: an operator that launches several line-length classes (a
+// ragged batch) puts its largest launch on the context's stream and the others
+// on side streams, concurrently, between a fork event and join events (the
+// pattern graph capture records as parallel branches), so the short launches
+// of small classes overlap the long one instead of queueing behind it.
+constexpr int kLwConcurrent = 4;
+constexpr int kLwFlags = kLwTwoTiles | kLwHaloWhole | kLwConcurrent;
+constexpr int kSideStreams = 3;
+struct LineStreams {
+  hipStream_t side[kSideStreams];
+  hipEvent_t fork;
+  hipEvent_t join[kSideStreams];
+};
 constexpr int kLdsTwoTiles = 80 * 1024;
 
 // LINE_IMPLICIT_EXPLICIT_KEEP / LINE_YZY_KEEP are LINE_IMPLICIT_EXPLICIT /
@@ -184,20 +196,20 @@ int line_window_default(int nmax);
 // The line kernels of one k-window family and state precision (a1..a3: the
 // z-step's Horner coefficients), one launch per non-empty line-length class.
 hipError_t line_w64_f64(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
-                        double a2, double a3, hipStream_t st);
+                        double a2, double a3, hipStream_t st, const LineStreams* ls);
 hipError_t line_w64_f32(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
-                        double a2, double a3, hipStream_t st);
+                        double a2, double a3, hipStream_t st, const LineStreams* ls);
 hipError_t line_w32_f64(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
-                        double a2, double a3, hipStream_t st);
+                        double a2, double a3, hipStream_t st, const LineStreams* ls);
 hipError_t line_w32_f32(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
-                        double a2, double a3, hipStream_t st);
+                        double a2, double a3, hipStream_t st, const LineStreams* ls);
 
 hipError_t launch_zrhs(const DevView& v, int prec, void* dphi, const void* phi, hipStream_t st);
 hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double c3, hipStream_t st);
 // c: sub-step of the line operators; dt_z, c3: the fused z-step (modes ZY, YZY);
 // lw: k-window family (64 or 32).
 hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, void* out, const void* in,
-                       double c, double dt_z, double c3, hipStream_t st);
+                       double c, double dt_z, double c3, hipStream_t st, const LineStreams* ls = nullptr);
 // Outlet sums, annulus level means and inlet values of one sub-step.
 // jacc (may be NULL): per-borehole outlet accumulator, += T_outlet of this sub-step.
 hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st, double* jacc = nullptr);

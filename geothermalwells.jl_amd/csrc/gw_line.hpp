@@ -116,7 +116,9 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
                                                           int64_t tile0, int64_t ntiles) {
   using MT = ModeTraits<MODE>;
   // The line is held in registers in the storage type; every recurrence runs
-  // in FP64 and rounds to the storage type only when it writes an element.
+  // in FP64.  In FP32-state kernels the values kept between the passes (the
+  // line, z_l = bd_l y_l, the solution) are FP32: each element is rounded
+  // twice in an implicit solve.
   using RT = TS;
   constexpr int S = Cfg::S, NM = Cfg::NMAX, LW = Cfg::LW, H = Cfg::H, G = Cfg::G;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -160,6 +162,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
   const int sid = sids[v.zone[S_.o_zone + kc]];
   const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : (sid == T.set[3] ? 3 : 0));
   constexpr int NL = (NM + Cfg::THREADS - 1) / Cfg::THREADS;  // staged elements per thread and array
+  const double cs_expl = sizeof(TS) == 4 ? c : 1.0;
   for (int q = 0; q < T.nset; ++q) {
     const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : (q == 2 ? T.set[2] : T.set[3]));
     const double* g = v.coef + v.set_off[sq];
@@ -175,8 +178,10 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
         cv[i][2] = ok ? g[SET_BU * N + l] : 0.0;
       }
       if (MT::expl) {
-        cv[i][3] = ok ? g[SET_EA * N + l] : 0.0;
-        cv[i][4] = ok ? g[SET_EB * N + l] : 0.0;
+        // FP32 lines stage the explicit step's products c ea, c eb; FP64 lines
+        // ea, eb (fewer registers in their explicit loop)
+        cv[i][3] = ok ? cs_expl * g[SET_EA * N + l] : 0.0;
+        cv[i][4] = ok ? cs_expl * g[SET_EB * N + l] : 0.0;
       }
     }
 #pragma unroll
@@ -269,7 +274,105 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
   // registers are otherwise short at three waves per SIMD).
   constexpr int KC = ((MT::halo && S > 40 && sizeof(TS) == 8) || (sizeof(TS) == 4 && Cfg::NW <= 4 && S > 32 && S <= 40)) ? 2 : 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
-  if (MT::impl) {
+  if constexpr (MT::impl && sizeof(TS) == 4) {
+    // Forward elimination y_l = b_l - fw_l y_{l-1}, then back substitution
+    // x_l = bd_l y_l - bu_l x_{l+1}, each in two passes over the segment:
+    // first the segment's map from its incoming to its outgoing value
+    // (v_out = V + M v_in, from zero carry, nothing stored), the carries by a
+    // G-step scan of those maps in LDS, then the exact recurrence from the
+    // incoming carry.  The forward pass stores z_l = bd_l y_l, so the back
+    // substitution is one FMA per element.
+    double y = 0.0, P = 1.0, wn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) wn[q] = fw[l0 + q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) wc[q] = wn[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) wn[q] = fw[l0 + r0 + KC + q];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        y = (double)d[r0 + q] - wc[q] * y;
+        P = -wc[q] * P;
+      }
+    }
+    shA[seg * LW + kk] = y;
+    shB[seg * LW + kk] = P;
+    __syncthreads();
+    y = 0.0;
+    for (int g = 0; g < seg; ++g) y = shA[g * LW + kk] + shB[g * LW + kk] * y;
+    __syncthreads();
+    double bn[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) { wn[q] = fw[l0 + q]; bn[q] = bd[l0 + q]; }
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double wc[KC], bc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) { wc[q] = wn[q]; bc[q] = bn[q]; }
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) { wn[q] = fw[l0 + r0 + KC + q]; bn[q] = bd[l0 + r0 + KC + q]; }
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        y = (double)d[r0 + q] - wc[q] * y;
+        d[r0 + q] = bc[q] * y;
+      }
+    }
+    double x = 0.0, Q = 1.0, un[KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) uc[q] = un[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - (r0 + KC + q)];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        x = (double)d[S - 1 - (r0 + q)] - uc[q] * x;
+        Q = -uc[q] * Q;
+      }
+    }
+    shA[seg * LW + kk] = x;
+    shB[seg * LW + kk] = Q;
+    __syncthreads();
+    x = 0.0;
+    for (int g = G - 1; g > seg; --g) x = shA[g * LW + kk] + shB[g * LW + kk] * x;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - q];
+#pragma unroll
+    for (int r0 = 0; r0 < S; r0 += KC) {
+      double uc[KC];
+#pragma unroll
+      for (int q = 0; q < KC; ++q) uc[q] = un[q];
+      if (r0 + KC < S) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - (r0 + KC + q)];
+      }
+#pragma unroll
+      for (int q = 0; q < KC; ++q) {
+        const int r = S - 1 - (r0 + q);
+        x = (double)d[r] - uc[q] * x;
+        d[r] = x;
+      }
+    }
+  }
+
+  if constexpr (MT::impl && sizeof(TS) == 8) {
+    // FP64 lines: each pass runs segment-locally from zero carry and is then
+    // corrected by the carried value times the running product of the
+    // multipliers (four passes; measured faster than the carry-first scheme
+    // for the FP64 register budgets).
     // forward: y_l = b_l - fw_l y_{l-1}
     auto wmul = [&](int l) { return fw[l]; };
     double y = 0.0, P = 1.0, wn[KC];
@@ -431,11 +534,12 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
   }
 
   if (MT::expl) {
-    // explicit half-step of the line direction: u + c (ea (u+ - u) - eb (u - u-))
+    // explicit half-step of the line direction: u + c (ea (u+ - u) - eb (u - u-)),
+    // FP32 lines with the products c ea, c eb staged in LDS
+    const TS cc = (TS)c;
     shA[seg * LW + kk] = (double)d[0];
     shB[seg * LW + kk] = (double)d[S - 1];
     __syncthreads();
-    const TS cc = (TS)c;
     TS prev = (seg > 0) ? (TS)shB[(seg - 1) * LW + kk] : (TS)d[0];
     const TS edge = (l0 + S < N) ? (TS)shA[(seg + 1) * LW + kk] : (TS)d[S - 1];
     TS an[KC], bn2[KC];
@@ -455,7 +559,11 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
         const int r = r0 + q;
         const TS x = (TS)d[r];
         const TS xp = (r < S - 1) ? (TS)d[r + 1] : edge;
-        const TS o = x + cc * (ac[q] * (xp - x) - bc2[q] * (x - prev));
+        TS o;
+        if constexpr (sizeof(TS) == 4)
+          o = x + ac[q] * (xp - x) - bc2[q] * (x - prev);
+        else
+          o = x + cc * (ac[q] * (xp - x) - bc2[q] * (x - prev));
         prev = x;
         d[r] = (RT)o;
       }
@@ -562,10 +670,12 @@ constexpr bool family_matches_classes() {
 static_assert(kLineClasses == 10, "one tile shape per line-length class");
 static_assert(family_matches_classes<64>() && family_matches_classes<32>(), "tile shapes follow the classes");
 
-// One launch per non-empty line-length class, each with its own tile shape.
+// One launch per non-empty line-length class, each with its own tile shape;
+// with side streams (ls) the largest launch stays on st and the others run
+// concurrently on the side streams between a fork and join events.
 template <int LW, class TS>
 hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const void* inp, double c, double a1,
-                       double a2, double a3, hipStream_t st) {
+                       double a2, double a3, hipStream_t st, const LineStreams* ls) {
   using F = LineFamily<LW>;
   TS* out = static_cast<TS*>(outp);
   const TS* in = static_cast<const TS*>(inp);
@@ -576,27 +686,46 @@ hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const vo
   // slot (most windows meet a single material zone), then the others; or
   // (halo_whole) each class as one range with its largest set count
   const bool split = halo && !v.halo_whole;
+  struct Launch { int q; int64_t t0, n; int hs; };
+  Launch L[2 * kLineClasses];
+  int nl = 0, big = 0;
   for (int q = 0; q < kLineClasses; ++q)
     for (int r = 0; r < (split ? 2 : 1); ++r) {
       const int64_t t0 = (split && r == 1) ? v.hone[f][q] : cls[q];
       const int64_t t1 = (split && r == 0) ? v.hone[f][q] : cls[q + 1];
-      const int64_t n = t1 - t0;
-      if (n == 0) continue;
-      const int hs = !halo ? kSlots : (split && r == 0 ? 1 : v.hslots[f][q]);
-      hipError_t e;
-      switch (q) {
-        case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-        default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, t0, n, hs, st); break;
-      }
-      if (e != hipSuccess) return e;
+      if (t1 == t0) continue;
+      L[nl] = Launch{q, t0, t1 - t0, !halo ? kSlots : (split && r == 0 ? 1 : v.hslots[f][q])};
+      if (L[nl].n > L[big].n) big = nl;
+      ++nl;
+    }
+  const bool conc = ls && nl > 1;
+  hipError_t e = hipSuccess;
+  if (conc) {
+    if ((e = hipEventRecord(ls->fork, st)) != hipSuccess) return e;
+    for (int s = 0; s < kSideStreams && s < nl - 1; ++s)
+      if ((e = hipStreamWaitEvent(ls->side[s], ls->fork, 0)) != hipSuccess) return e;
+  }
+  for (int i = 0, k = 0; i < nl; ++i) {
+    const Launch& x = L[i];
+    const hipStream_t s = (!conc || i == big) ? st : ls->side[(k++) % kSideStreams];
+    switch (x.q) {
+      case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+    }
+    if (e != hipSuccess) return e;
+  }
+  if (conc)
+    for (int s = 0; s < kSideStreams && s < nl - 1; ++s) {
+      if ((e = hipEventRecord(ls->join[s], ls->side[s])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(st, ls->join[s], 0)) != hipSuccess) return e;
     }
   return hipSuccess;
 }

@@ -4,8 +4,8 @@
 namespace gw {
 
 hipError_t line_w32_f32(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
-                        double a2, double a3, hipStream_t st) {
-  return line_family<32, float>(v, mode, dir, out, in, c, a1, a2, a3, st);
+                        double a2, double a3, hipStream_t st, const LineStreams* ls) {
+  return line_family<32, float>(v, mode, dir, out, in, c, a1, a2, a3, st, ls);
 }
 
 }  // namespace gw

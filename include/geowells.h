@@ -43,8 +43,11 @@ enum { GW_F64 = 0, GW_F32_STATE = 1 };
  * / _W32 force one width; with one of them, GW_FLAG_LINE_TWO_TILES pads each
  * tile's LDS so at most two share a CU, and GW_FLAG_HALO_WHOLE launches the
  * fused z-step tiles of a line class as one range (default: split by their
- * coefficient-set count).  Results do not depend on these choices. */
-enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2, GW_FLAG_LINE_TWO_TILES = 4, GW_FLAG_HALO_WHOLE = 8 };
+ * coefficient-set count); GW_FLAG_LINE_CONCURRENT runs the launches of the
+ * line-length classes of a batch concurrently on side streams.  Results do not
+ * depend on these choices. */
+enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2, GW_FLAG_LINE_TWO_TILES = 4, GW_FLAG_HALO_WHOLE = 8,
+       GW_FLAG_LINE_CONCURRENT = 16 };
 
 /* Inlet models (src/physics.jl:283-318). */
 enum {

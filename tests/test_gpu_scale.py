@@ -218,8 +218,9 @@ def test_array_3x3_prefix():
     assert np.abs(T - T_ref).max() <= 1e-6
 
 
-@pytest.mark.parametrize("line_window", LINE_WINDOWS + [None])
-def test_batched_line_classes(line_window):
+@pytest.mark.parametrize("line_window,concurrent", [(64, False), (32, False), (None, False), (64, True),
+                                                   (32, True)])
+def test_batched_line_classes(line_window, concurrent):
     """One batch with a scenario in every line-length class (each class
     launches its own tile shape), two steps, against the oracle per scenario."""
     torch = _torch()
@@ -229,7 +230,7 @@ def test_batched_line_classes(line_window):
     classes = {next(q for q, m in enumerate((48, 64, 96, 128, 160, 192, 256, 320, 416, 640)) if len(w["gridx"]) <= m)
                for w in ws}
     assert classes == set(range(10)), classes
-    cache = g.create_batch_cache([_kw(w) for w in ws], line_window=line_window)
+    cache = g.create_batch_cache([_kw(w) for w in ws], line_window=line_window, line_concurrent=concurrent)
     u = cache.new_state()
     refs = [_oracle_steps(("one", len(w["gridx"])), w, 2, cache.rock2_c3) for w in ws]
     for s, (u0, _, _) in enumerate(refs):
@@ -240,3 +241,26 @@ def test_batched_line_classes(line_window):
         got = cache.scenario_view(u, s).cpu().numpy()
         assert H.rel_err(got, ref) <= 1e-10, (s, got.shape)
         assert abs(T[s] - T_ref[0]) <= 1e-6, s
+
+
+def test_batched_concurrent_launches_in_graph_replays():
+    """The batch of every line class stepped through captured step graphs
+    (several replays, two calls): concurrent class launches on side streams
+    give the same bits as sequential ones."""
+    torch = _torch()
+    g = H.gw()
+    ws = [_long_line_case(n, nbh=1) if n <= 60 else _class_case(n)
+          for n in (40, 60, 90, 120, 150, 180, 240, 300, 400, 600)]
+    out = []
+    for conc in (False, True):
+        cache = g.create_batch_cache([_kw(w) for w in ws], line_window=64, line_concurrent=conc)
+        u = cache.new_state()
+        for s, w in enumerate(ws):
+            orc = H.oracle_for(w)
+            cache.scenario_view(u, s).copy_(torch.from_numpy(orc.initial_condition(2.29, 0.3)))
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 120.0, 5))
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 600.0, 120.0, 4))
+        cache.sync()
+        out.append((u.clone(), cache.T_outlet))
+    assert torch.equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
