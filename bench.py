@@ -10,8 +10,9 @@ hipGraph per step.
 
 Besides the headline line the JSON carries, as extra objects: `batched` (64
 ragged single-borehole scenarios one at a time in FP64 against all of them in
-one context in FP64 and FP32), `large_fp32` (the 5x5 array, 601 x 601 x 2002
-cells, FP32 state against FP64) and `adjoint` (the layered single borehole,
+one context in FP64, FP32 and FP32 with stochastic rounding), `large_fp32`
+(the 5x5 array, 601 x 601 x 2002 cells, both FP32 states against FP64) and
+`adjoint` (the layered single borehole,
 121 x 121 x 602 cells: the 30-day outlet gradient's cost in forward steps and
 central-difference checks); each can be switched off by its `--*-steps 0`.
 `roofline` also gives the rate of a plain device copy of the field measured
@@ -188,8 +189,9 @@ def _timed_steps(gw, cache, u, dt, warmup, steps):
 def bench_batched(gw, steps, warmup, err_steps):
     """The 64 ragged single-borehole scenarios: one scenario at a time in FP64
     (a context each) vs all 64 in one context (every launch covers all of
-    them) in FP64 and in FP32 state; the FP32 outlet error is taken against
-    the batched FP64 run of identical inputs after the same steps."""
+    them) in FP64 and in FP32 state (rounded to nearest, and stochastically);
+    the FP32 outlet errors are taken against the batched FP64 run of
+    identical inputs after the same steps."""
     import torch
     from geothermalwells_jl_amd import workloads as W
     ws = W.c2()
@@ -209,7 +211,7 @@ def bench_batched(gw, steps, warmup, err_steps):
     res["plain_fp64_one_at_a_time"] = total * steps / t_plain
     outs = {}
     n_err = max(err_steps, warmup + steps)
-    for prec, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+    for prec, dtype in (("f64", torch.float64), ("f32", torch.float32), ("f32sr", torch.float32)):
         cache = gw.create_batch_cache([W.scenario_kwargs(w) for w in ws], precision=prec)
         u = _init_state(gw, cache, ws, dtype)
         t = _timed_steps(gw, cache, u, dt, warmup, steps)
@@ -222,16 +224,18 @@ def bench_batched(gw, steps, warmup, err_steps):
     res["outlet_err_after_steps"] = n_err
     res["speedup_batched_fp32_vs_plain_fp64"] = res["batched_f32"] / res["plain_fp64_one_at_a_time"]
     res["speedup_batched_fp64_vs_plain_fp64"] = res["batched_f64"] / res["plain_fp64_one_at_a_time"]
+    res["speedup_batched_fp32sr_vs_plain_fp64"] = res["batched_f32sr"] / res["plain_fp64_one_at_a_time"]
     res["outlet_max_abs_err_fp32_vs_fp64_K"] = float(np.abs(outs["f32"] - outs["f64"]).max())
+    res["outlet_max_abs_err_fp32sr_vs_fp64_K"] = float(np.abs(outs["f32sr"] - outs["f64"]).max())
     res["unit"] = "cell-steps/s"
     return res
 
 
 def bench_large(gw, steps, warmup, err_steps):
     """The 5x5 array (601 x 601 x 2002 cells, dt 600 s): FP32 state (FP64
-    Thomas elimination and outlet reductions) against FP64, cell-steps/s of
-    each and the FP32 outlet error after err_steps steps from the same
-    initial field."""
+    Thomas elimination and outlet reductions; rounded to nearest, and with
+    stochastic rounding) against FP64, cell-steps/s of each and the FP32
+    outlet errors after err_steps steps from the same initial field."""
     import torch
     from geothermalwells_jl_amd import workloads as W
     w = W.c4()
@@ -241,7 +245,7 @@ def bench_large(gw, steps, warmup, err_steps):
            "steps": steps, "unit": "cell-steps/s"}
     outs = {}
     n_err = max(err_steps, warmup + steps)
-    for prec, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+    for prec, dtype in (("f64", torch.float64), ("f32", torch.float32), ("f32sr", torch.float32)):
         cache = gw.create_cache(precision=prec, **W.scenario_kwargs(w))
         u = _init_state(gw, cache, [w], dtype)
         t = _timed_steps(gw, cache, u, dt, warmup, steps)
@@ -253,8 +257,10 @@ def bench_large(gw, steps, warmup, err_steps):
         del cache, u
         torch.cuda.empty_cache()
     res["speedup_f32_vs_f64"] = res["f32"] / res["f64"]
+    res["speedup_f32sr_vs_f64"] = res["f32sr"] / res["f64"]
     res["outlet_err_after_steps"] = n_err
     res["outlet_max_abs_err_fp32_vs_fp64_K"] = float(np.abs(outs["f32"] - outs["f64"]).max())
+    res["outlet_max_abs_err_fp32sr_vs_fp64_K"] = float(np.abs(outs["f32sr"] - outs["f64"]).max())
     return res
 
 
@@ -348,8 +354,8 @@ def run_horizon(gw, name, precisions, dt, every, out_path):
     precision, the contexts stepped in alternating chunks of `every` steps
     (default one simulated day).  At every chunk end: the outlet
     temperatures and the field's range.  Prints one JSON line with the
-    largest and final outlet difference of FP32 against FP64 (when both ran)
-    and writes the series to out_path (.npz) when given."""
+    largest and final outlet difference of each FP32 precision against FP64
+    (when both ran) and writes the series to out_path (.npz) when given."""
     import torch
     from geothermalwells_jl_amd import workloads as W
     if name == "c2":
@@ -391,10 +397,11 @@ def run_horizon(gw, name, precisions, dt, every, out_path):
         T = np.array(r["T"])
         res[prec] = {"outlet_final": [float(x) for x in T[-1]], "field_min": min(r["lo"]), "field_max": max(r["hi"]),
                      "finite": bool(np.all(np.isfinite(T)))}
-    if "f64" in runs and "f32" in runs:
-        d = np.abs(np.array(runs["f32"]["T"]) - np.array(runs["f64"]["T"]))
-        res["outlet_err_fp32_vs_fp64_max_K"] = float(d.max())
-        res["outlet_err_fp32_vs_fp64_final_K"] = float(d[-1].max())
+    for prec in runs:
+        if prec != "f64" and "f64" in runs:
+            d = np.abs(np.array(runs[prec]["T"]) - np.array(runs["f64"]["T"]))
+            res[f"outlet_err_{prec}_vs_f64_max_K"] = float(d.max())
+            res[f"outlet_err_{prec}_vs_f64_final_K"] = float(d[-1].max())
     if out_path:
         arrays = {"t": dt * every * np.arange(1, len(next(iter(runs.values()))["T"]) + 1)}
         for prec, r in runs.items():
@@ -406,16 +413,20 @@ def run_horizon(gw, name, precisions, dt, every, out_path):
 
 
 def compare_horizons(path_a, path_b):
-    """Outlet difference between the FP32 series of one horizon file and the
-    FP64 series of another (runs made in separate calls)."""
+    """Outlet difference between each FP32 series (f32, f32sr) of one horizon
+    file and the FP64 series of another (runs made in separate calls)."""
     a, b = np.load(path_a, allow_pickle=False), np.load(path_b, allow_pickle=False)
-    Ta = a["T_f32"] if "T_f32" in a else a["T_f64"]
-    Tb = b["T_f64"] if "T_f64" in b else b["T_f32"]
-    if Ta.shape != Tb.shape or not np.array_equal(a["t"], b["t"]):
+    Tb = b["T_f64"]
+    if not np.array_equal(a["t"], b["t"]):
         raise ValueError("the two series are not sampled at the same times")
-    d = np.abs(Ta - Tb)
-    return {"samples": int(len(a["t"])), "t_end_s": float(a["t"][-1]), "outlet_err_max_K": float(d.max()),
-            "outlet_err_final_K": float(d[-1].max()), "outlet_err_max_at_s": float(a["t"][int(d.max(axis=1).argmax())])}
+    res = {"samples": int(len(a["t"])), "t_end_s": float(a["t"][-1])}
+    for prec in ("f32", "f32sr"):
+        if f"T_{prec}" not in a:
+            continue
+        d = np.abs(a[f"T_{prec}"] - Tb)
+        res[prec] = {"outlet_err_max_K": float(d.max()), "outlet_err_final_K": float(d[-1].max()),
+                     "outlet_err_max_at_s": float(a["t"][int(d.max(axis=1).argmax())])}
+    return res
 
 
 def _cpu_share():

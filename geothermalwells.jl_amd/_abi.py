@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgeowells_hip.so")
 
 GW_OK, GW_EINVAL, GW_EHIP, GW_ENOMEM, GW_ESTATE = 0, -1, -2, -3, -4
-GW_F64, GW_F32_STATE = 0, 1
+GW_F64, GW_F32_STATE, GW_F32_STATE_SR = 0, 1, 2
 GW_FLAG_LINE_W64, GW_FLAG_LINE_W32, GW_FLAG_LINE_TWO_TILES, GW_FLAG_HALO_WHOLE = 1, 2, 4, 8
 GW_FLAG_LINE_CONCURRENT = 16
 GW_INLET_CONSTANT, GW_INLET_HEAT_EXCHANGER, GW_INLET_PER_WELL, GW_INLET_HOST = 0, 1, 2, 3

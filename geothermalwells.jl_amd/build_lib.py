@@ -38,7 +38,8 @@ def _run_all(cmds):
 # host units: set-up, C ABI, grid generators (no floating-point contraction)
 HOST_UNITS = ["gw_host.cpp", "gw_grids.cpp"]
 KERNEL_UNITS = ["gw_kernels.hip", "gw_line_w64_f64.hip", "gw_line_w64_f32.hip", "gw_line_w32_f64.hip",
-                "gw_line_w32_f32.hip", "gw_line_adj_w64.hip", "gw_line_adj_w32.hip", "gw_adjoint.hip", "gw_post.hip"]
+                "gw_line_w32_f32.hip", "gw_line_w64_f32sr.hip", "gw_line_w32_f32sr.hip", "gw_line_adj_w64.hip",
+                "gw_line_adj_w32.hip", "gw_adjoint.hip", "gw_post.hip"]
 
 
 def build(verbose=True):

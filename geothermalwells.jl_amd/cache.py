@@ -83,6 +83,12 @@ class _Scenario:
         return (len(self.gridx), len(self.gridy), len(self.gridz))
 
 
+# State precisions: "f64"; "f32" (FP32 field, FP64 Thomas recurrences and
+# outlet reductions, round to nearest); "f32sr" (the same with stochastic
+# rounding of every state update: unbiased over long runs).
+PRECISIONS = {"f64": _abi.GW_F64, "f32": _abi.GW_F32_STATE, "f32sr": _abi.GW_F32_STATE_SR}
+
+
 class Cache:
     """The `cache` (the ODEProblem parameter `p`) of one or more scenarios.
 
@@ -96,6 +102,8 @@ class Cache:
         self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
         if self.backend.type != "cuda":
             raise ValueError("libgeowells_hip runs on a GPU device only")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision is one of {sorted(PRECISIONS)}")
         self.precision = precision
         self.dtype = torch.float64 if precision == "f64" else torch.float32
         self.rock2_c3 = float(rock2_c3)
@@ -105,7 +113,7 @@ class Cache:
         prob = _abi.GwProblem()
         prob.n_scen = len(scenarios)
         prob.scen = ctypes.cast(arr, ctypes.POINTER(_abi.GwScenario))
-        prob.precision = _abi.GW_F64 if precision == "f64" else _abi.GW_F32_STATE
+        prob.precision = PRECISIONS[precision]
         prob.stream = stream
         prob.rock2_c3 = self.rock2_c3
         if line_window not in (None, 64, 32):
@@ -209,7 +217,8 @@ class Cache:
 def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inlet_model, precision="f64",
                  stream=None, rock2_c3=DEFAULT_ROCK2_C3, line_window=None, line_two_tiles=False,
                  line_halo_whole=False):
-    """create_cache (src/cache.jl:55-116).  `line_window` forces the k-window
+    """create_cache (src/cache.jl:55-116).  `precision`: "f64" (the
+    reference's), "f32" or "f32sr" (FP32 state, see PRECISIONS).  `line_window` forces the k-window
     width (64 or 32 lanes) of the line kernels, and with it `line_two_tiles`
     (LDS padded to two tiles per CU) and `line_halo_whole` (fused z-step tiles
     of a class in one launch); None lets the first multi-step call time the

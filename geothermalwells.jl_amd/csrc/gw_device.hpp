@@ -111,6 +111,69 @@ GW_DEV double level_sum(int n, F ld, double* red) {
   return r;
 }
 
+// ---------------------------------------------------------------------------
+// Stochastic rounding FP64 -> FP32 (GW_F32_STATE_SR).  A double's low 29
+// mantissa bits are what FP32 drops; adding a uniform 29-bit integer to the
+// bit pattern and clearing those bits rounds the magnitude up with
+// probability (dropped bits) / 2^29, so E[result] = x (the carry into bit 29
+// moves to the next float, across a binade too).  r is a 24-bit uniform
+// number, placed at the top of the 29 bits (the probability is exact to
+// 2^-24).  The result is an FP32 value, so the final conversion is exact;
+// values FP32 holds exactly are returned unchanged.
+// ---------------------------------------------------------------------------
+GW_DEV float sr_f32(double x, uint32_t r24) {
+  const uint64_t b = (__builtin_bit_cast(uint64_t, x) + ((uint64_t)r24 << 5)) & ~(uint64_t)0x1FFFFFFFu;
+  return (float)__builtin_bit_cast(double, b);
+}
+
+// 32-bit integer hash (a bijection; "lowbias32", C. Wellons' hash prospector).
+__host__ __device__ constexpr uint32_t lowbias32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// The random numbers of a line's elements: the states of the 24-bit LCG
+// s <- (kLcgA s + kLcgC) mod 2^24 (full period: a = 1 mod 4, c odd; one
+// 24-bit multiply-add per number), element l taking the l-th state.
+// lcg_jump(n) is the map of n steps, s -> a s + c (mod 2^24); kLcgAInv steps
+// back.
+constexpr uint32_t kLcgA = 16598013u, kLcgC = 12820163u, kLcgMask = 0xFFFFFFu;
+struct LcgJump {
+  uint32_t a, c;
+};
+__host__ __device__ constexpr LcgJump lcg_jump(uint32_t n) {
+  uint32_t A = 1u, C = 0u, ma = kLcgA, mc = kLcgC;
+  while (n) {
+    if (n & 1u) { A *= ma; C = C * ma + mc; }
+    mc = mc * (ma + 1u);
+    ma *= ma;
+    n >>= 1;
+  }
+  return LcgJump{A & kLcgMask, C & kLcgMask};
+}
+__host__ __device__ constexpr uint32_t lcg_inverse_a() {
+  uint32_t x = kLcgA;  // correct to 3 bits (a odd), each Newton step doubles them
+  for (int i = 0; i < 5; ++i) x *= 2u - kLcgA * x;
+  return x & kLcgMask;
+}
+constexpr uint32_t kLcgAInv = lcg_inverse_a();
+static_assert(((kLcgA * kLcgAInv) & kLcgMask) == 1u, "LCG multiplier inverse");
+static_assert(((lcg_jump(3).a * 5u + lcg_jump(3).c) & kLcgMask) ==
+                  ((kLcgA * (kLcgA * (kLcgA * 5u + kLcgC) + kLcgC) + kLcgC) & kLcgMask), "LCG jump");
+GW_DEV uint32_t lcg_next(uint32_t s) { return (__umul24(s, kLcgA) + kLcgC) & kLcgMask; }
+GW_DEV uint32_t lcg_prev(uint32_t s) { return __umul24((s - kLcgC) & kLcgMask, kLcgAInv) & kLcgMask; }
+GW_DEV uint32_t lcg_apply(LcgJump j, uint32_t s) { return (__umul24(s, j.a) + j.c) & kLcgMask; }
+
+// Random-number stream of one rounding point (p) of step n for the element
+// with flat index idx (a line's first element: the line's stream start).
+GW_DEV uint32_t sr_stream(uint32_t idx, uint32_t n, uint32_t p) {
+  return lowbias32(idx ^ lowbias32(n * 8u + p)) & kLcgMask;
+}
+
 // Scenario s with prefix[s] <= idx < prefix[s+1].
 GW_DEV int find_scen(const int64_t* prefix, int n, int64_t idx) {
   int lo = 0, hi = n - 1;

@@ -249,6 +249,7 @@ struct gw_ctx {
   int32_t* d_setid = nullptr;
   PipeCol* d_pcol = nullptr;
   double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
+  uint32_t* d_step_ctr = nullptr;  // GW_F32_STATE_SR: step index keying the stochastic rounding
   int nx_max = 0, ny_max = 0, nz_max = 0, countz_max = 0;
   int64_t cells_total = 0, cols_total = 0;
   // tile offsets of the line-length classes, per k-window family
@@ -336,6 +337,7 @@ struct gw_ctx {
              v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
     v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size();
     v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
+    v.step_ctr = d_step_ctr;
     v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
     v.cells_total = cells_total; v.cols_total = cols_total;
     return v;
@@ -795,6 +797,23 @@ int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* j
   return hip_status(launch_pipe_advect(v, ctx.prec, phi, ctx.stream), "pipe_advect");
 }
 
+// GW_F32_STATE_SR: the stochastic rounding of step n draws random numbers
+// keyed by n (the device counter d_step_ctr): a gw_steps call sets it to the
+// index of its first step, t / dt, and every step advances it on the device
+// (inside the step graphs too), so a run split into several calls rounds like
+// one call.  The separate operator entry points advance it after each call,
+// so repeated calls draw new numbers.
+int set_step_ctr(gw_ctx& ctx, double t, double dt) {
+  if (ctx.prec != GW_F32_STATE_SR) return GW_OK;
+  const double n = std::nearbyint(t / dt);
+  const uint32_t v = (uint32_t)(uint64_t)(int64_t)(std::isfinite(n) ? std::fmod(n, 4294967296.0) : 0.0);
+  return hip_status(launch_step_ctr(ctx.d_step_ctr, true, v, ctx.stream), "step counter");
+}
+int advance_step_ctr(gw_ctx& ctx) {
+  if (ctx.prec != GW_F32_STATE_SR) return GW_OK;
+  return hip_status(launch_step_ctr(ctx.d_step_ctr, false, 0u, ctx.stream), "step counter");
+}
+
 // ADI_and_ADV_callback! with the tables of c = dt/2 (scratch_mem: see
 // sub_step_advect).
 int adi_adv_tab(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, double* scratch_mem) {
@@ -811,7 +830,9 @@ int adi_adv_tab(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, do
   rc = sub_step_advect(ctx, tab, phi, t + dt / 2, nullptr, scratch_mem);  // :458
   if (rc) return rc;
   // implicit y  (:461)
-  return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
+  rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), phi, phi, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
+  if (rc) return rc;
+  return advance_step_ctr(ctx);
 }
 
 int adi_adv(gw_ctx& ctx, void* phi, double t, double dt) {
@@ -851,15 +872,18 @@ int check_zstep(gw_ctx& ctx, double dt) {
 }
 
 int zstep(gw_ctx& ctx, void* phi, double dt) {
-  const int rc = check_zstep(ctx, dt);
+  int rc = check_zstep(ctx, dt);
   if (rc) return rc;
-  return hip_status(launch_zstep(ctx.view(nullptr), ctx.prec, phi, dt, ctx.c3, ctx.stream), "zstep");
+  rc = hip_status(launch_zstep(ctx.view(nullptr), ctx.prec, phi, dt, ctx.c3, ctx.stream), "zstep");
+  if (rc) return rc;
+  return advance_step_ctr(ctx);
 }
 
 int create_impl(const gw_problem* p, gw_ctx** out) {
   if (!p || !out) return fail(GW_EINVAL, "null argument");
   if (p->n_scen <= 0 || !p->scen) return fail(GW_EINVAL, "no scenarios");
-  if (p->precision != GW_F64 && p->precision != GW_F32_STATE) return fail(GW_EINVAL, "unknown precision");
+  if (p->precision != GW_F64 && p->precision != GW_F32_STATE && p->precision != GW_F32_STATE_SR)
+    return fail(GW_EINVAL, "unknown precision");
   const int32_t width = p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32);
   const int32_t variant = p->flags & (GW_FLAG_LINE_TWO_TILES | GW_FLAG_HALO_WHOLE | GW_FLAG_LINE_CONCURRENT);
   if ((p->flags & ~(width | variant)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) || (variant && !width))
@@ -1045,6 +1069,7 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     std::vector<double> zeros(std::max(ctx->nbh_total, 1), 0.0);
     ctx->d_tout = ctx->up(zeros);
     ctx->d_inlet = ctx->up(zeros);
+    ctx->d_step_ctr = ctx->up(std::vector<uint32_t>(1, 0u));
     ctx->d_inlet_tab = ctx->up(inlet_tab);
   } catch (HipFail& e) {
     return fail(GW_EHIP, std::string("device set-up failed: ") + hipGetErrorString(e.e));
@@ -1163,9 +1188,12 @@ int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end
   rc = sub_step_advect(ctx, tab, cur, t_end + c, jacc);
   if (rc) return rc;
   if (last)
-    return hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
-  return hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, ctx.lw(LINE_YZY, 1), next, cur, c, dt, ctx.c3, ctx.stream, ctx.lsp()),
+    rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
+  else
+    rc = hip_status(launch_line(v, ctx.prec, LINE_YZY, 1, ctx.lw(LINE_YZY, 1), next, cur, c, dt, ctx.c3, ctx.stream, ctx.lsp()),
                     "implicit y + z-step + explicit y");
+  if (rc) return rc;
+  return advance_step_ctr(ctx);
 }
 
 // Line-tile autotune: each line operator of the fused pipeline is timed in
@@ -1272,6 +1300,7 @@ int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
   int rc = prepare_steps(*ctx, phi, dt, tab);
   if (rc) return rc;
   if (!tab->adv_err.empty()) return fail(GW_EINVAL, tab->adv_err);
+  if ((rc = set_step_ctr(*ctx, t, dt))) return rc;
   void* B = ctx->buf2;
   rc = fused_zy(*ctx, tab, phi, B, dt);
   if (rc) return rc;
@@ -1311,8 +1340,10 @@ int gw_line_op(gw_ctx* ctx, void* out, const void* in, double c, double dt_z, in
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, ctx->lw(mode, dir), out, in, c, dt_z, ctx->c3, ctx->stream, ctx->lsp()),
-                    "line operator");
+  rc = hip_status(launch_line(ctx->view(tab), ctx->prec, mode, dir, ctx->lw(mode, dir), out, in, c, dt_z, ctx->c3, ctx->stream, ctx->lsp()),
+                  "line operator");
+  if (rc) return rc;
+  return advance_step_ctr(*ctx);
 }
 
 int gw_outlet(gw_ctx* ctx, double* T_out) {
@@ -1348,7 +1379,9 @@ int gw_explicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, ctx->lw(LINE_EXPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "explicit");
+  rc = hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_EXPLICIT, dir, ctx->lw(LINE_EXPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "explicit");
+  if (rc) return rc;
+  return advance_step_ctr(*ctx);
 }
 
 int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
@@ -1356,7 +1389,9 @@ int gw_implicit(gw_ctx* ctx, void* out, const void* in, double c, int32_t dir) {
   CTab* tab = nullptr;
   int rc = build_ctab(*ctx, c, tab);
   if (rc) return rc;
-  return hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, ctx->lw(LINE_IMPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "implicit");
+  rc = hip_status(launch_line(ctx->view(tab), ctx->prec, LINE_IMPLICIT, dir, ctx->lw(LINE_IMPLICIT, dir), out, in, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "implicit");
+  if (rc) return rc;
+  return advance_step_ctr(*ctx);
 }
 
 int gw_advect(gw_ctx* ctx, void* phi, double c, double t) {
@@ -1437,7 +1472,7 @@ int gw_pipe_level_means(gw_ctx* ctx, const void* phi, int32_t s, double* T_inner
 int gw_extract_x_profile(const void* T, int32_t precision, int32_t nx, int32_t ny, int32_t nz, const double* gridy,
                          const double* gridz, double z_target, double yc, double* profile, void* stream) {
   if (!T || !gridy || !gridz || !profile || nx <= 0 || ny <= 0 || nz <= 0 ||
-      (precision != GW_F64 && precision != GW_F32_STATE))
+      (precision != GW_F64 && precision != GW_F32_STATE && precision != GW_F32_STATE_SR))
     return fail(GW_EINVAL, "bad argument");
   // y index closest to yc (the first of equals), src/util.jl:276
   int j = 0;
@@ -1501,6 +1536,7 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
   };
   do {
     void* B = ctx->buf2;
+    if ((rc = set_step_ctr(*ctx, t, dt))) break;
     if ((rc = rec(0)) || (rc = fused_zy(*ctx, tab, phi, B, dt)) || (rc = rec(1))) break;
     if ((rc = hip_status(hipEventSynchronize(ev[1]), "event sync")) || (rc = acc(0, 1, 0))) break;
     for (int n = 0; n < nsteps && rc == GW_OK; ++n) {
@@ -1515,7 +1551,7 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
       if ((rc = sub_step_advect(*ctx, tab, cur, te + c)) || (rc = rec(3))) break;
       rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, ctx->lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "y")
                 : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, ctx->lw(LINE_YZY, 1), next, cur, c, dt, ctx->c3, ctx->stream, ctx->lsp()), "yzy");
-      if (rc || (rc = rec(4))) break;
+      if (rc || (rc = rec(4)) || (rc = advance_step_ctr(*ctx))) break;
       if ((rc = hip_status(hipEventSynchronize(ev[4]), "event sync"))) break;
       if ((rc = acc(0, 1, 1)) || (rc = acc(1, 2, 2)) || (rc = acc(2, 3, 3)) || (rc = acc(3, 4, last ? 5 : 4))) break;
     }

@@ -9,6 +9,8 @@
 //
 // Every kernel is bandwidth-bound (no MFMA).  Lanes always run along k, the
 // contiguous axis, so each wave touches 64 consecutive elements (512 B in FP64).
+#include <type_traits>
+
 #include "gw_device.hpp"
 
 namespace gw {
@@ -42,10 +44,15 @@ __global__ __launch_bounds__(256) void k_zrhs(DevView v, TS* __restrict__ dphi,
 // column; the column lives in registers (R values per thread), the stage
 // vector in LDS.
 // ---------------------------------------------------------------------------
-template <class TS, int R>
-__global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, TS a1, TS a2, TS a3) {
+// SR (FP32 state, GW_F32_STATE_SR): the stages run in FP64 and the result is
+// rounded stochastically, element k of column g drawing
+// lowbias32(flat index ^ key(step, point 6)).
+template <class TS, int R, bool SR>
+__global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, double a1_, double a2_, double a3_) {
+  using AT = typename std::conditional<SR, double, TS>::type;  // arithmetic type
+  const AT a1 = (AT)a1_, a2 = (AT)a2_, a3 = (AT)a3_;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  TS* w = reinterpret_cast<TS*>(smem_raw);
+  AT* w = reinterpret_cast<AT*>(smem_raw);
   const int64_t g = blockIdx.x;
   const int s = find_scen(v.col_prefix, v.nscen, g);
   const ScenDev S = v.sc[s];
@@ -55,31 +62,31 @@ __global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, 
   const double* zc = v.zcoef + 2 * (S.o_zcoef + (int64_t)c * nz);
   TS* p = phi + S.off + col * nz;
   const int t = threadIdx.x;
-  TS u[R], ea[R], eb[R], nw[R];
+  AT u[R], ea[R], eb[R], nw[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = t + 256 * r;
     if (k < nz) {
-      u[r] = p[k];
+      u[r] = (AT)p[k];
       const double2 e = *reinterpret_cast<const double2*>(zc + 2 * k);
-      ea[r] = (TS)e.x;
-      eb[r] = (TS)e.y;
+      ea[r] = (AT)e.x;
+      eb[r] = (AT)e.y;
       w[k] = u[r];
     } else {
       u[r] = 0; ea[r] = 0; eb[r] = 0;
     }
   }
   __syncthreads();
-  const TS alpha[3] = {a1, a2, a3};
+  const AT alpha[3] = {a1, a2, a3};
 #pragma unroll
   for (int st = 0; st < 3; ++st) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = t + 256 * r;
       if (k < nz) {
-        const TS w0 = w[k];
-        const TS wm = (k > 0) ? w[k - 1] : w0;
-        const TS wp = (k + 1 < nz) ? w[k + 1] : w0;
+        const AT w0 = w[k];
+        const AT wm = (k > 0) ? w[k - 1] : w0;
+        const AT wp = (k + 1 < nz) ? w[k + 1] : w0;
         nw[r] = u[r] + alpha[st] * (ea[r] * (wp - w0) - eb[r] * (w0 - wm));
       }
     }
@@ -93,10 +100,17 @@ __global__ __launch_bounds__(256) void k_zstep(DevView v, TS* __restrict__ phi, 
       __syncthreads();
     }
   }
+  uint32_t key = 0u;
+  if constexpr (SR) key = lowbias32(*v.step_ctr * 8u + 6u);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = t + 256 * r;
-    if (k < nz) p[k] = nw[r];
+    if (k < nz) {
+      if constexpr (SR)
+        p[k] = sr_f32(nw[r], lowbias32((uint32_t)(S.off + col * nz + k) ^ key) >> 8);
+      else
+        p[k] = (TS)nw[r];
+    }
   }
 }
 
@@ -312,24 +326,35 @@ hipError_t launch_zrhs(const DevView& v, int prec, void* dphi, const void* phi, 
   return hipGetLastError();
 }
 
-template <class TS>
+template <class TS, bool SR>
 static hipError_t zstep_t(const DevView& v, TS* phi, double dt, double c3, hipStream_t st) {
   const dim3 grid((unsigned)v.cols_total), block(256);
-  const size_t sh = (size_t)v.nz_max * sizeof(TS);
-  const TS a1 = (TS)(2.0 * c3 * dt), a2 = (TS)(0.5 * dt), a3 = (TS)dt;
+  const size_t sh = (size_t)v.nz_max * (SR ? sizeof(double) : sizeof(TS));
+  // Horner coefficients rounded to the arithmetic type
+  const double a1 = 2.0 * c3 * dt, a2 = 0.5 * dt, a3 = dt;
   const int R = (v.nz_max + 255) / 256;
-  if (R <= 1) hipLaunchKernelGGL((k_zstep<TS, 1>), grid, block, sh, st, v, phi, a1, a2, a3);
-  else if (R <= 2) hipLaunchKernelGGL((k_zstep<TS, 2>), grid, block, sh, st, v, phi, a1, a2, a3);
-  else if (R <= 4) hipLaunchKernelGGL((k_zstep<TS, 4>), grid, block, sh, st, v, phi, a1, a2, a3);
-  else if (R <= 8) hipLaunchKernelGGL((k_zstep<TS, 8>), grid, block, sh, st, v, phi, a1, a2, a3);
-  else if (R <= 16) hipLaunchKernelGGL((k_zstep<TS, 16>), grid, block, sh, st, v, phi, a1, a2, a3);
+  if (R <= 1) hipLaunchKernelGGL((k_zstep<TS, 1, SR>), grid, block, sh, st, v, phi, a1, a2, a3);
+  else if (R <= 2) hipLaunchKernelGGL((k_zstep<TS, 2, SR>), grid, block, sh, st, v, phi, a1, a2, a3);
+  else if (R <= 4) hipLaunchKernelGGL((k_zstep<TS, 4, SR>), grid, block, sh, st, v, phi, a1, a2, a3);
+  else if (R <= 8) hipLaunchKernelGGL((k_zstep<TS, 8, SR>), grid, block, sh, st, v, phi, a1, a2, a3);
+  else if (R <= 16) hipLaunchKernelGGL((k_zstep<TS, 16, SR>), grid, block, sh, st, v, phi, a1, a2, a3);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
 hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double c3, hipStream_t st) {
-  if (prec == 0) return zstep_t<double>(v, (double*)phi, dt, c3, st);
-  return zstep_t<float>(v, (float*)phi, dt, c3, st);
+  if (prec == 0) return zstep_t<double, false>(v, (double*)phi, dt, c3, st);
+  if (prec == 2) return zstep_t<float, true>(v, (float*)phi, dt, c3, st);
+  return zstep_t<float, false>(v, (float*)phi, dt, c3, st);
+}
+
+__global__ void k_step_ctr(uint32_t* ctr, int set, uint32_t value) {
+  if (threadIdx.x == 0) ctr[0] = set ? value : ctr[0] + 1u;
+}
+
+hipError_t launch_step_ctr(uint32_t* ctr, bool set, uint32_t value, hipStream_t st) {
+  hipLaunchKernelGGL(k_step_ctr, dim3(1), dim3(64), 0, st, ctr, set ? 1 : 0, value);
+  return hipGetLastError();
 }
 
 int line_window_default(int nmax) { return nmax <= kLineClassMax[8] ? 64 : 32; }
@@ -344,9 +369,11 @@ hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, vo
   const int w = lw & ~kLwFlags;
   if (w == 64)
     return prec == 0 ? line_w64_f64(v2, mode, dir, out, in, c, a1, a2, a3, st, side)
+         : prec == 2 ? line_w64_f32sr(v2, mode, dir, out, in, c, a1, a2, a3, st, side)
                      : line_w64_f32(v2, mode, dir, out, in, c, a1, a2, a3, st, side);
   if (w == 32)
     return prec == 0 ? line_w32_f64(v2, mode, dir, out, in, c, a1, a2, a3, st, side)
+         : prec == 2 ? line_w32_f32sr(v2, mode, dir, out, in, c, a1, a2, a3, st, side)
                      : line_w32_f32(v2, mode, dir, out, in, c, a1, a2, a3, st, side);
   return hipErrorInvalidValue;
 }

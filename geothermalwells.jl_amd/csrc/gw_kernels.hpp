@@ -65,6 +65,10 @@ struct DevView {
   int32_t lds_floor;            // bytes: each line tile requests at least this much LDS
   int32_t halo_whole;           // 1: the fused z-step modes launch each class as one range
   int32_t zc_cls;               // column classes whose z coefficients a halo tile stages (most of any scenario)
+  // GW_F32_STATE_SR: the step index that keys the random numbers of the
+  // stochastic rounding (set by the host at a call's first step, advanced on
+  // the device after each step)
+  uint32_t* step_ctr;
 };
 
 // The k-window argument of launch_line is 64 or 32, optionally with
@@ -203,6 +207,13 @@ hipError_t line_w32_f64(const DevView& v, int mode, int dir, void* out, const vo
                         double a2, double a3, hipStream_t st, const LineStreams* ls);
 hipError_t line_w32_f32(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
                         double a2, double a3, hipStream_t st, const LineStreams* ls);
+// FP32 state with stochastic rounding of every state update (GW_F32_STATE_SR)
+hipError_t line_w64_f32sr(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
+                          double a2, double a3, hipStream_t st, const LineStreams* ls);
+hipError_t line_w32_f32sr(const DevView& v, int mode, int dir, void* out, const void* in, double c, double a1,
+                          double a2, double a3, hipStream_t st, const LineStreams* ls);
+// *ctr = value (set) or *ctr += 1, by a one-thread kernel (graph-capturable)
+hipError_t launch_step_ctr(uint32_t* ctr, bool set, uint32_t value, hipStream_t st);
 
 hipError_t launch_zrhs(const DevView& v, int prec, void* dphi, const void* phi, hipStream_t st);
 hipError_t launch_zstep(const DevView& v, int prec, void* phi, double dt, double c3, hipStream_t st);

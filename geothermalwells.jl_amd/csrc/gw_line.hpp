@@ -103,18 +103,25 @@ GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
 // registers (S <= 52), eight-wave tiles two to a CU when it fits in 128.  An
 // FP32 line takes half the registers: four-wave FP32 tiles aim at five per CU
 // (96 registers) up to S = 32 and four (128) up to S = 48.
-template <class Cfg, class TS, int MODE>
+template <class Cfg, class TS, int MODE, bool SR = false>
 constexpr int line_min_waves() {
   if (sizeof(TS) == 4 && Cfg::NW <= 4 && Cfg::S > 32 && Cfg::S <= 40) return 4;
+  // the stochastic-rounding 8-wave 40-element tiles: four per SIMD (128
+  // registers, a few spilled) run faster than three without spills
+  if (SR && Cfg::NW == 8 && Cfg::S == 40) return 4;
   if (MODE == LINE_IMPLICIT_EXPLICIT || MODE == LINE_ZY) return 2;
   return Cfg::NW <= 4 ? (Cfg::S <= 52 ? 3 : 2) : (Cfg::S <= 32 ? 4 : 2);
 }
 
-template <class Cfg, class TS, int MODE>
-__global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) void k_line(DevView v, int dir, TS* out, const TS* in,
+// SR (FP32 state only): every rounding of an updated value to FP32 is
+// stochastic (sr_f32): the implicit solve's two stores, the fused z-step's
+// result and the explicit step's result; round to nearest otherwise.
+template <class Cfg, class TS, int MODE, bool SR>
+__global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())) void k_line(DevView v, int dir, TS* out, const TS* in,
                                                           double c, double a1, double a2, double a3,
                                                           int64_t tile0, int64_t ntiles) {
   using MT = ModeTraits<MODE>;
+  static_assert(!SR || sizeof(TS) == 4, "stochastic rounding of an FP32 state");
   // The line is held in registers in the storage type; every recurrence runs
   // in FP64.  In FP32-state kernels the values kept between the passes (the
   // line, z_l = bd_l y_l, the solution) are FP32: each element is rounded
@@ -247,6 +254,24 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
 
   RT d[S];
   const int l0 = seg * S;
+  // Stochastic rounding: rounding point p (0, 1: the implicit solve's
+  // forward and backward stores, 2: the z-step, 3: the explicit step; + 4 on
+  // y-lines) of step n draws, for element l of this line at level kc, the
+  // l-th state of the stream sr_stream(first element's flat index, n, p).
+  // Every tile that holds the element (as output or as halo) therefore draws
+  // the same number whatever its tile shape, and a run split into several
+  // calls rounds exactly like one call.
+  uint32_t sr_idx = 0u, sr_n = 0u;
+  LcgJump sr_j{1u, 0u};
+  if constexpr (SR) {
+    sr_n = *v.step_ctr;
+    sr_idx = (uint32_t)(S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kc);
+    constexpr LcgJump js = lcg_jump(S);  // to element l0 = seg * S
+    for (int g = 0; g < seg; ++g) sr_j = LcgJump{(sr_j.a * js.a) & kLcgMask, (sr_j.c * js.a + js.c) & kLcgMask};
+  }
+  auto sr_first = [&](uint32_t n, int p) { return lcg_apply(sr_j, sr_stream(sr_idx, n, (uint32_t)(dir * 4 + p))); };
+  // the z-step and explicit y of YZY belong to the next step
+  const uint32_t sr_n2 = MODE == LINE_YZY ? sr_n + 1u : sr_n;
   const int64_t ustride = uniform64(stride);
   const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
   // element r of the lane's segment: field base + an element offset kept in
@@ -271,8 +296,11 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
   // Every element loop below runs in chunks of KC elements: the LDS operands
   // of the next chunk are fetched while the current one computes.  The long
   // FP64 segments of the fused z-step modes use chunks of two (their
-  // registers are otherwise short at three waves per SIMD).
-  constexpr int KC = ((MT::halo && S > 40 && sizeof(TS) == 8) || (sizeof(TS) == 4 && Cfg::NW <= 4 && S > 32 && S <= 40)) ? 2 : 4;
+  // registers are otherwise short at three waves per SIMD), and so do the
+  // stochastic-rounding fused z-step 8-wave 40-element tiles (at four waves
+  // per SIMD).
+  constexpr int KC = ((MT::halo && S > 40 && sizeof(TS) == 8) || (sizeof(TS) == 4 && Cfg::NW <= 4 && S > 32 && S <= 40) ||
+                      (SR && MT::halo && Cfg::NW == 8 && S == 40)) ? 2 : 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
   if constexpr (MT::impl && sizeof(TS) == 4) {
     // Forward elimination y_l = b_l - fw_l y_{l-1}, then back substitution
@@ -307,6 +335,8 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
     for (int g = 0; g < seg; ++g) y = shA[g * LW + kk] + shB[g * LW + kk] * y;
     __syncthreads();
     double bn[KC];
+    uint32_t rs = 0u;
+    if constexpr (SR) rs = sr_first(sr_n, 0);
 #pragma unroll
     for (int q = 0; q < KC; ++q) { wn[q] = fw[l0 + q]; bn[q] = bd[l0 + q]; }
 #pragma unroll
@@ -321,7 +351,12 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
 #pragma unroll
       for (int q = 0; q < KC; ++q) {
         y = (double)d[r0 + q] - wc[q] * y;
-        d[r0 + q] = bc[q] * y;
+        if constexpr (SR) {
+          d[r0 + q] = sr_f32(bc[q] * y, rs);
+          rs = lcg_next(rs);
+        } else {
+          d[r0 + q] = bc[q] * y;
+        }
       }
     }
     double x = 0.0, Q = 1.0, un[KC];
@@ -348,6 +383,10 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
     x = 0.0;
     for (int g = G - 1; g > seg; --g) x = shA[g * LW + kk] + shB[g * LW + kk] * x;
     __syncthreads();
+    if constexpr (SR) {
+      constexpr LcgJump jb = lcg_jump(S - 1);  // the segment's last element, stepped back from there
+      rs = lcg_apply(jb, sr_first(sr_n, 1));
+    }
 #pragma unroll
     for (int q = 0; q < KC; ++q) un[q] = bu[l0 + S - 1 - q];
 #pragma unroll
@@ -363,7 +402,12 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
       for (int q = 0; q < KC; ++q) {
         const int r = S - 1 - (r0 + q);
         x = (double)d[r] - uc[q] * x;
-        d[r] = x;
+        if constexpr (SR) {
+          d[r] = sr_f32(x, rs);
+          rs = lcg_prev(rs);
+        } else {
+          d[r] = x;
+        }
       }
     }
   }
@@ -492,6 +536,8 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
     // neighbours k-1 / k+1 come from lanes -1 / +1 by DPP wave shifts (the
     // window edge lanes keep their own value: they are halo, never output)
     constexpr int KZ = KC;
+    uint32_t rz = 0u;
+    if constexpr (SR) rz = sr_first(sr_n2, 2);
     TS ean[KZ], ebn[KZ];
 #pragma unroll
     for (int q = 0; q < KZ; ++q) {
@@ -515,21 +561,50 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
       // the chunk's KZ elements advance through the three stages together
       // (KZ independent chains per stage); the stage loop stays a real loop:
       // unrolled, the compiler interleaves chunks and the line spills
-      TS u0[KZ], w[KZ];
-#pragma unroll
-      for (int q = 0; q < KZ; ++q) { u0[q] = (TS)d[r0 + q]; w[q] = u0[q]; }
-#pragma unroll 1
-      for (int st = 0; st < 3; ++st) {
-        const TS al = st == 0 ? (TS)a1 : (st == 1 ? (TS)a2 : (TS)a3);
+      if constexpr (SR) {
+        // stages in increments: w = u0 + e, e <- al (ea (du+ + e+ - e) -
+        // eb (du- + e - e-)) with the differences du of the loaded values
+        // (exact), so no stage rounds u0 + e; one stochastic rounding at the end
+        TS u0[KZ], dp[KZ], dm[KZ], e[KZ];
 #pragma unroll
         for (int q = 0; q < KZ; ++q) {
-          const TS wm = wave_shr1(w[q]);
-          const TS wp = wave_shl1(w[q]);
-          w[q] = u0[q] + al * (eac[q] * (wp - w[q]) - ebc[q] * (w[q] - wm));
+          u0[q] = (TS)d[r0 + q];
+          dp[q] = wave_shl1(u0[q]) - u0[q];
+          dm[q] = u0[q] - wave_shr1(u0[q]);
+          e[q] = (TS)0;
         }
-      }
+#pragma unroll 1
+        for (int st = 0; st < 3; ++st) {
+          const TS al = st == 0 ? (TS)a1 : (st == 1 ? (TS)a2 : (TS)a3);
 #pragma unroll
-      for (int q = 0; q < KZ; ++q) d[r0 + q] = (RT)w[q];
+          for (int q = 0; q < KZ; ++q) {
+            const TS em = wave_shr1(e[q]);
+            const TS ep = wave_shl1(e[q]);
+            e[q] = al * (eac[q] * (dp[q] + (ep - e[q])) - ebc[q] * (dm[q] + (e[q] - em)));
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) {
+          d[r0 + q] = sr_f32((double)u0[q] + (double)e[q], rz);
+          rz = lcg_next(rz);
+        }
+      } else {
+        TS u0[KZ], w[KZ];
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) { u0[q] = (TS)d[r0 + q]; w[q] = u0[q]; }
+#pragma unroll 1
+        for (int st = 0; st < 3; ++st) {
+          const TS al = st == 0 ? (TS)a1 : (st == 1 ? (TS)a2 : (TS)a3);
+#pragma unroll
+          for (int q = 0; q < KZ; ++q) {
+            const TS wm = wave_shr1(w[q]);
+            const TS wp = wave_shl1(w[q]);
+            w[q] = u0[q] + al * (eac[q] * (wp - w[q]) - ebc[q] * (w[q] - wm));
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < KZ; ++q) d[r0 + q] = (RT)w[q];
+      }
     }
   }
 
@@ -537,6 +612,8 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
     // explicit half-step of the line direction: u + c (ea (u+ - u) - eb (u - u-)),
     // FP32 lines with the products c ea, c eb staged in LDS
     const TS cc = (TS)c;
+    uint32_t rx = 0u;
+    if constexpr (SR) rx = sr_first(sr_n2, 3);
     shA[seg * LW + kk] = (double)d[0];
     shB[seg * LW + kk] = (double)d[S - 1];
     __syncthreads();
@@ -560,10 +637,14 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
         const TS x = (TS)d[r];
         const TS xp = (r < S - 1) ? (TS)d[r + 1] : edge;
         TS o;
-        if constexpr (sizeof(TS) == 4)
+        if constexpr (SR) {
+          o = sr_f32((double)x + (double)(ac[q] * (xp - x) - bc2[q] * (x - prev)), rx);
+          rx = lcg_next(rx);
+        } else if constexpr (sizeof(TS) == 4) {
           o = x + ac[q] * (xp - x) - bc2[q] * (x - prev);
-        else
+        } else {
           o = x + cc * (ac[q] * (xp - x) - bc2[q] * (x - prev));
+        }
         prev = x;
         d[r] = (RT)o;
       }
@@ -583,38 +664,38 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE>())) vo
 }
 
 
-template <class Cfg, class TS, int MODE>
+template <class Cfg, class TS, int MODE, bool SR>
 static void line_launch(const DevView& v, int dir, TS* out, const TS* in, double c, double a1, double a2,
                         double a3, int64_t tile0, int64_t ntiles, int slots, hipStream_t st) {
   const size_t sh = std::max(line_lds_doubles<Cfg, TS, MODE>(slots, v.zc_cls) * sizeof(double), (size_t)v.lds_floor);
-  hipLaunchKernelGGL((k_line<Cfg, TS, MODE>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out, in,
-                     c, a1, a2, a3, tile0, ntiles);
+  hipLaunchKernelGGL((k_line<Cfg, TS, MODE, SR>), dim3((unsigned)ntiles), dim3(Cfg::THREADS), sh, st, v, dir, out,
+                     in, c, a1, a2, a3, tile0, ntiles);
 }
 
-template <class Cfg, class TS>
+template <class Cfg, class TS, bool SR>
 static hipError_t line_cfg(const DevView& v, int mode, int dir, TS* out, const TS* in, double c, double a1,
                            double a2, double a3, int64_t tile0, int64_t ntiles, int hslots, hipStream_t st) {
   switch (mode) {
     case LINE_EXPLICIT:
-      line_launch<Cfg, TS, LINE_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+      line_launch<Cfg, TS, LINE_EXPLICIT, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
       break;
     case LINE_IMPLICIT:
-      line_launch<Cfg, TS, LINE_IMPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+      line_launch<Cfg, TS, LINE_IMPLICIT, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
       break;
     case LINE_IMPLICIT_EXPLICIT:
-      line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+      line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
       break;
-    case LINE_ZY: line_launch<Cfg, TS, LINE_ZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
-    case LINE_YZY: line_launch<Cfg, TS, LINE_YZY>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
+    case LINE_ZY: line_launch<Cfg, TS, LINE_ZY, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
+    case LINE_YZY: line_launch<Cfg, TS, LINE_YZY, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st); break;
     case LINE_IMPLICIT_EXPLICIT_KEEP:
       if constexpr (std::is_same<TS, double>::value) {
-        line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT_KEEP>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
+        line_launch<Cfg, TS, LINE_IMPLICIT_EXPLICIT_KEEP, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, kSlots, st);
         break;
       }
       return hipErrorInvalidValue;
     case LINE_YZY_KEEP:
       if constexpr (std::is_same<TS, double>::value) {
-        line_launch<Cfg, TS, LINE_YZY_KEEP>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st);
+        line_launch<Cfg, TS, LINE_YZY_KEEP, SR>(v, dir, out, in, c, a1, a2, a3, tile0, ntiles, hslots, st);
         break;
       }
       return hipErrorInvalidValue;
@@ -673,7 +754,7 @@ static_assert(family_matches_classes<64>() && family_matches_classes<32>(), "til
 // One launch per non-empty line-length class, each with its own tile shape;
 // with side streams (ls) the largest launch stays on st and the others run
 // concurrently on the side streams between a fork and join events.
-template <int LW, class TS>
+template <int LW, class TS, bool SR = false>
 hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const void* inp, double c, double a1,
                        double a2, double a3, hipStream_t st, const LineStreams* ls) {
   using F = LineFamily<LW>;
@@ -709,16 +790,16 @@ hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const vo
     const Launch& x = L[i];
     const hipStream_t s = (!conc || i == big) ? st : ls->side[(k++) % kSideStreams];
     switch (x.q) {
-      case 0: e = line_cfg<typename F::C0>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 1: e = line_cfg<typename F::C1>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 2: e = line_cfg<typename F::C2>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 3: e = line_cfg<typename F::C3>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 4: e = line_cfg<typename F::C4>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 5: e = line_cfg<typename F::C5>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 6: e = line_cfg<typename F::C6>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 7: e = line_cfg<typename F::C7>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      case 8: e = line_cfg<typename F::C8>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
-      default: e = line_cfg<typename F::C9>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 0: e = line_cfg<typename F::C0, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 1: e = line_cfg<typename F::C1, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 2: e = line_cfg<typename F::C2, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 3: e = line_cfg<typename F::C3, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 4: e = line_cfg<typename F::C4, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 5: e = line_cfg<typename F::C5, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 6: e = line_cfg<typename F::C6, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 7: e = line_cfg<typename F::C7, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      case 8: e = line_cfg<typename F::C8, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
+      default: e = line_cfg<typename F::C9, TS, SR>(v, mode, dir, out, in, c, a1, a2, a3, x.t0, x.n, x.hs, s); break;
     }
     if (e != hipSuccess) return e;
   }

@@ -11,7 +11,7 @@
  *    message for the calling thread.  Nothing aborts or throws across the ABI.
  *  - State buffers are device pointers owned by the caller, valid only during
  *    the call.  A field is the reference's column-major phi[k, j, i] of size
- *    (Nz, Ny, Nx), element type double (GW_F64) or float (GW_F32_STATE).  A
+ *    (Nz, Ny, Nx), element type double (GW_F64) or float (GW_F32_STATE*).  A
  *    batch packs its scenarios back to back (gw_scenario_offset).
  *  - A context is bound to one HIP stream and is not re-entrant.  Calls are
  *    stream-ordered and asynchronous except gw_outlet, gw_create and a
@@ -35,7 +35,17 @@ typedef enum gw_status {
   GW_ESTATE = -4   /* call not valid in the context's current state */
 } gw_status;
 
-enum { GW_F64 = 0, GW_F32_STATE = 1 };
+/* State precision.  GW_F32_STATE: FP32 field, stencils and advection in FP32,
+ * the Thomas recurrences and the outlet reductions in FP64 (each element
+ * rounded to nearest).  GW_F32_STATE_SR: the same FP32 field, with every
+ * rounding of an updated value (the implicit solves' stores, the z-step's and
+ * the explicit steps' results, computed as FP32 increments added in FP64)
+ * stochastic, so sub-ulp changes are kept in expectation instead of lost: no
+ * counterpart in the reference (which is FP64); for runs long enough that
+ * round-to-nearest stagnation biases the FP32 field.  Its random numbers are
+ * keyed by the step index t / dt, the element and the rounding point: results
+ * are reproducible, and a run split into gw_steps calls equals one call. */
+enum { GW_F64 = 0, GW_F32_STATE = 1, GW_F32_STATE_SR = 2 };
 
 /* gw_problem.flags.  The line kernels come in two k-window widths (64 and 32
  * lanes) and launch variants; by default the first gw_steps call times them
@@ -99,7 +109,7 @@ typedef struct gw_scenario {
 typedef struct gw_problem {
   int32_t n_scen;
   const gw_scenario *scen;
-  int32_t precision;   /* GW_F64 or GW_F32_STATE */
+  int32_t precision;   /* GW_F64, GW_F32_STATE or GW_F32_STATE_SR */
   void *stream;        /* hipStream_t; NULL: the context creates its own */
   double rock2_c3;     /* cubic coefficient of the minimum-degree ROCK2 polynomial; <= 0: default 1/16 */
   int32_t flags;       /* GW_FLAG_*, or 0 */

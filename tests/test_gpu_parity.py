@@ -223,7 +223,7 @@ def test_golden_example_gpu():
         assert np.linalg.norm(got[i, j] - u0[i, j]) <= 1e-5 * np.linalg.norm(u0[i, j])
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32sr"])
 @pytest.mark.parametrize("window", [64, 32])
 @pytest.mark.parametrize("two_tiles", [False, True])
 @pytest.mark.parametrize("halo_whole", [False, True])

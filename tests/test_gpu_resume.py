@@ -31,7 +31,7 @@ def _steps(g, cache, u, t0, dt, n):
     cache.sync()
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "f32sr"])
 @pytest.mark.parametrize("window", [64, 32])
 def test_split_run_equals_uninterrupted(tmp_path, precision, window):
     import torch
