@@ -762,12 +762,16 @@ double stability_bound(double c3) {
 // memory used for the reduction partials and level means instead of the
 // tables' own buffers.
 int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* jacc = nullptr,
-                    double* scratch_mem = nullptr) {
+                    double* scratch_mem = nullptr, double* hist = nullptr) {
   if (!t->adv_err.empty()) return fail(GW_EINVAL, t->adv_err);
   DevView v = ctx.view(t);
   if (scratch_mem) {
     v.partial = scratch_mem;
     v.means = scratch_mem + t->npartial;
+  }
+  if (hist) {  // keep this sub-step's level means and inlets (sub_step_replay)
+    v.means = hist;
+    v.inlet = hist + t->nmeans;
   }
   int rc = hip_status(launch_pipe_reduce(v, ctx.prec, phi, ctx.stream, jacc), "pipe_reduce");
   if (rc) return rc;
@@ -794,6 +798,16 @@ int sub_step_advect(gw_ctx& ctx, const CTab* t, void* phi, double tau, double* j
     rc = hip_status(hipStreamSynchronize(ctx.stream), "sync");
     if (rc) return rc;
   }
+  return hip_status(launch_pipe_advect(v, ctx.prec, phi, ctx.stream), "pipe_advect");
+}
+
+// The advection of a sub-step whose level means and inlets an earlier
+// sub_step_advect kept in hist (the same state: the adjoint's recompute),
+// without the reductions.
+int sub_step_replay(gw_ctx& ctx, const CTab* t, void* phi, const double* hist) {
+  DevView v = ctx.view(t);
+  v.means = const_cast<double*>(hist);
+  v.inlet = const_cast<double*>(hist) + t->nmeans;
   return hip_status(launch_pipe_advect(v, ctx.prec, phi, ctx.stream), "pipe_advect");
 }
 
@@ -1177,15 +1191,18 @@ int fused_zy(gw_ctx& ctx, const CTab* tab, const void* in, void* out, double dt)
 
 // One step's advection/x part (in place on `cur`) and its closing y kernel
 // from `cur` into `next` (YZY, or the plain implicit y when `last`).
+// hist (may be NULL): keeps the two sub-steps' level means and inlets at hist
+// and hist + (nmeans + nbh).
 int fused_rest(gw_ctx& ctx, const CTab* tab, void* cur, void* next, double t_end, double dt, bool last,
-               double* jacc = nullptr) {
+               double* jacc = nullptr, double* hist = nullptr) {
   const double c = dt / 2;
   DevView v = ctx.view(tab);
-  int rc = sub_step_advect(ctx, tab, cur, t_end);
+  const size_t hs = (size_t)tab->nmeans + (size_t)ctx.nbh_total;
+  int rc = sub_step_advect(ctx, tab, cur, t_end, nullptr, nullptr, hist);
   if (rc) return rc;
   rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT_EXPLICIT, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "x");
   if (rc) return rc;
-  rc = sub_step_advect(ctx, tab, cur, t_end + c, jacc);
+  rc = sub_step_advect(ctx, tab, cur, t_end + c, jacc, nullptr, hist ? hist + hs : nullptr);
   if (rc) return rc;
   if (last)
     rc = hip_status(launch_line(v, ctx.prec, LINE_IMPLICIT, 1, ctx.lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit y");
@@ -1734,16 +1751,21 @@ AdjPipe build_adj_pipe(const gw_ctx& ctx, const CTab& tab, DevBufs& db) {
 }
 
 // n fused steps from t on phi (as gw_steps, without graphs), adding every
-// step's second-sub-step outlet to jacc.
-int run_fused(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, int n, double* jacc) {
+// step's second-sub-step outlet to jacc; hist (may be NULL): every sub-step's
+// level means and inlets, 2 (nmeans + nbh) doubles per step.
+int run_fused(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, int n, double* jacc, double* hist) {
   void* B = ctx.buf2;
+  const size_t hs = 2 * ((size_t)tab->nmeans + (size_t)ctx.nbh_total);
   int rc = fused_zy(ctx, tab, phi, B, dt);
   if (rc) return rc;
   for (int k = 1; k < n; ++k) {
     const bool odd = (k & 1) != 0;
-    if ((rc = fused_rest(ctx, tab, odd ? B : phi, odd ? phi : B, t + k * dt, dt, false, jacc))) return rc;
+    if ((rc = fused_rest(ctx, tab, odd ? B : phi, odd ? phi : B, t + k * dt, dt, false, jacc,
+                         hist ? hist + (size_t)(k - 1) * hs : nullptr)))
+      return rc;
   }
-  return fused_rest(ctx, tab, (n & 1) ? B : phi, phi, t + n * dt, dt, true, jacc);
+  return fused_rest(ctx, tab, (n & 1) ? B : phi, phi, t + n * dt, dt, true, jacc,
+                    hist ? hist + (size_t)(n - 1) * hs : nullptr);
 }
 
 // The n steps of a segment from u_0 = U[0]: the z-step + explicit y of the
@@ -1751,21 +1773,27 @@ int run_fused(gw_ctx& ctx, const CTab* tab, void* phi, double t, double dt, int 
 // kernel keeping e_m = (I - cA_x)^-1 b in E[m], and the implicit y keeping
 // u_{m+1} in U[m+1], fused with the next step's z-step + explicit y except
 // after the last step.  wa, wb: work fields (the fused y kernel writes out of
-// place).
+// place).  hist (may be NULL): the forward run's level means and inlets of
+// the segment's sub-steps (run_fused): the advection then skips the
+// reductions, which would recompute the same values.
 int adj_recompute(gw_ctx& ctx, const CTab* tab, double* const* U, double* const* E, int n, double* wa, double* wb,
-                  double t_first_end, double dt) {
+                  double t_first_end, double dt, const double* hist) {
   const double c = dt / 2;
   const DevView v = ctx.view(tab);
+  const size_t hs = (size_t)tab->nmeans + (size_t)ctx.nbh_total;
+  auto advect = [&](int m, int sub, double t_end) {
+    return hist ? sub_step_replay(ctx, tab, wa, hist + (2 * (size_t)m + sub) * hs) : sub_step_advect(ctx, tab, wa, t_end);
+  };
   int rc = hip_status(launch_line(v, GW_F64, LINE_ZY, 1, ctx.lw(LINE_ZY, 1), wa, U[0], c, dt, ctx.c3, ctx.stream, ctx.lsp()),
                       "z-step + explicit y");
   for (int m = 0; m < n && !rc; ++m) {
     const double t_end = t_first_end + (double)m * dt;
-    rc = sub_step_advect(ctx, tab, wa, t_end);
+    rc = advect(m, 0, t_end);
     DevView vk = v;
     vk.aux = E[m];
     if (!rc) rc = hip_status(launch_line(vk, GW_F64, LINE_IMPLICIT_EXPLICIT_KEEP, 0, ctx.lw(LINE_IMPLICIT_EXPLICIT, 0),
                                          wa, wa, c, 0.0, 0.0, ctx.stream, ctx.lsp()), "implicit + explicit x");
-    if (!rc) rc = sub_step_advect(ctx, tab, wa, t_end + c);
+    if (!rc) rc = advect(m, 1, t_end + c);
     if (rc) break;
     if (m + 1 < n) {
       DevView vy = v;
@@ -1921,13 +1949,20 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     double* WB = db.alloc<double>(cells);
     double* WB2 = db.alloc<double>(cells);
     double* jacc = db.up(std::vector<double>(nbh, 0.0));
+    // every sub-step's level means and inlets of the forward run, for the
+    // recompute (up to 1 GiB; beyond that the recompute reduces again)
+    const size_t hstep = 2 * ((size_t)tab->nmeans + (size_t)nbh);
+    double* hist = hstep * (size_t)nsteps * sizeof(double) <= ((size_t)1 << 30)
+                       ? db.alloc<double>(hstep * (size_t)nsteps) : nullptr;
     // forward with checkpoints
     if ((rc = hip_status(hipMemcpyAsync(U[0], phi0, bytes, hipMemcpyDeviceToDevice, C.stream), "copy"))) return rc;
     for (int sg = 0; sg < nseg; ++sg) {
       const int n = std::min(seg, nsteps - sg * seg);
       if ((rc = hip_status(hipMemcpyAsync(ck[sg], U[0], bytes, hipMemcpyDeviceToDevice, C.stream), "copy")))
         return rc;
-      if ((rc = run_fused(C, tab, U[0], t0 + (double)sg * seg * dt, dt, n, jacc))) return rc;
+      if ((rc = run_fused(C, tab, U[0], t0 + (double)sg * seg * dt, dt, n, jacc,
+                          hist ? hist + (size_t)sg * seg * hstep : nullptr)))
+        return rc;
     }
     std::vector<double> w(nbh), hj(nbh);
     for (int b = 0; b < nbh; ++b) w[b] = w_bh ? w_bh[b] : 1.0 / nbh;
@@ -1964,7 +1999,9 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
       const int base = sg * seg;
       if ((rc = hip_status(hipMemcpyAsync(U[0], ck[sg], bytes, hipMemcpyDeviceToDevice, C.stream), "copy")))
         return rc;
-      if ((rc = adj_recompute(C, tab, U.data(), E.data(), n, WB, WB2, t0 + (double)(base + 1) * dt, dt))) return rc;
+      if ((rc = adj_recompute(C, tab, U.data(), E.data(), n, WB, WB2, t0 + (double)(base + 1) * dt, dt,
+                              hist ? hist + (size_t)base * hstep : nullptr)))
+        return rc;
       for (int m = n - 1; m >= 0; --m) {
         if ((rc = adj_line(ADJ_Y, Bf, A, U[m + 1]))) return rc;
         if ((rc = hip_status(launch_adj_pipe(v, ag, ap, Bf, seed, C.stream), "adjoint advection"))) return rc;
