@@ -80,7 +80,7 @@ def profile_traffic(mode):
                 continue
             if col is None or len(cells) != len(col) or not cells[0].startswith("`void k_line<"):
                 continue
-            if not cells[0].endswith(f"double, {mode}>`"):
+            if not (cells[0].endswith(f"double, {mode}>`") or cells[0].endswith(f"double, {mode}, false>`")):
                 continue
             name = cells[0].strip("`")
             b = (float(cells[col["HBM read MiB (scaled)"]]) + float(cells[col["HBM write MiB (scaled)"]])) * 2 ** 20

@@ -100,9 +100,10 @@ GW_DEV int64_t xcd_remap(int64_t b, int64_t n) {
 
 // Waves per SIMD the register budget aims at: a tile of NW waves puts NW/4 on
 // each SIMD, so four-wave tiles run three to a CU when the line fits in 168
-// registers (S <= 52), eight-wave tiles two to a CU when it fits in 128.  An
-// FP32 line takes half the registers: four-wave FP32 tiles aim at five per CU
-// (96 registers) up to S = 32 and four (128) up to S = 48.
+// registers (S <= 52), eight-wave tiles two to a CU when it fits in 128
+// (sixteen-wave tiles, 1024 threads, one).  An FP32 line takes half the
+// registers: four-wave FP32 tiles aim at five per CU (96 registers) up to
+// S = 32 and four (128) up to S = 48.
 template <class Cfg, class TS, int MODE, bool SR = false>
 constexpr int line_min_waves() {
   if (sizeof(TS) == 4 && Cfg::NW <= 4 && Cfg::S > 32 && Cfg::S <= 40) return 4;
@@ -298,9 +299,9 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   // FP64 segments of the fused z-step modes use chunks of two (their
   // registers are otherwise short at three waves per SIMD), and so do the
   // stochastic-rounding fused z-step 8-wave 40-element tiles (at four waves
-  // per SIMD).
+  // per SIMD), and segments whose length is not a multiple of four.
   constexpr int KC = ((MT::halo && S > 40 && sizeof(TS) == 8) || (sizeof(TS) == 4 && Cfg::NW <= 4 && S > 32 && S <= 40) ||
-                      (SR && MT::halo && Cfg::NW == 8 && S == 40)) ? 2 : 4;
+                      (SR && MT::halo && Cfg::NW == 8 && S == 40) || S % 4 != 0) ? 2 : 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
   if constexpr (MT::impl && sizeof(TS) == 4) {
     // Forward elimination y_l = b_l - fw_l y_{l-1}, then back substitution
@@ -724,7 +725,7 @@ template <> struct LineFamily<64> {
   using C5 = LineCfg<4, 48, 64>;
   using C6 = LineCfg<8, 32, 64>;
   using C7 = LineCfg<8, 40, 64>;
-  using C8 = LineCfg<8, 52, 64>;
+  using C8 = LineCfg<16, 26, 64>;
   using C9 = LineCfg<8, 80, 64>;
 };
 template <> struct LineFamily<32> {
@@ -736,7 +737,7 @@ template <> struct LineFamily<32> {
   using C5 = LineCfg<2, 48, 32>;
   using C6 = LineCfg<4, 32, 32>;
   using C7 = LineCfg<4, 40, 32>;
-  using C8 = LineCfg<4, 52, 32>;
+  using C8 = LineCfg<8, 26, 32>;
   using C9 = LineCfg<8, 40, 32>;
 };
 

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
                                                            double c, double a1, double a2, double a3,
                                                            int64_t tile0, int64_t ntiles) {
   using AT = AdjTraits<MODE>;
-  constexpr int S = Cfg::S, NM = Cfg::NMAX, LW = Cfg::LW, H = Cfg::H, G = Cfg::G, KC = 4;
+  constexpr int S = Cfg::S, NM = Cfg::NMAX, LW = Cfg::LW, H = Cfg::H, G = Cfg::G, KC = S % 4 == 0 ? 4 : 2;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW]
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
     // x in chunks of KG elements, the next chunk's loads in flight while the
     // current one computes; the barrier keeps the compiler from hoisting every
     // load of the segment (which would spill the line)
-    constexpr int KG = S % 8 == 0 ? 8 : 4;
+    constexpr int KG = S % 8 == 0 ? 8 : (S % 4 == 0 ? 4 : 2);
     const double xafter = (l0 + S < N) ? *lane_ptr(x + soff(useg + (int64_t)S * ustride), kb) : 0.0;
     double xn[KG];
 #pragma unroll
