@@ -33,6 +33,15 @@ GW_DEV void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// LDS DMA (global_load_lds_dwordx4): each active lane copies the 16 bytes at
+// g straight into LDS, lane i to lds_wave + 16 i (lds_wave wave-uniform), no
+// registers involved.  It completes under vmcnt; a __syncthreads() after it
+// waits for it (the barrier's fence waits for vmcnt(0) while LDS DMA is
+// pending).
+GW_DEV void lds_dma16(const void* g, void* lds_wave) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave, 16, 0, 0);
+}
+
 // A wave-uniform element offset pinned in SGPRs: base + offset + lane byte
 // offset then uses the scalar-base + 32-bit vector-offset addressing form.
 GW_DEV int64_t soff(int64_t x) {

@@ -202,6 +202,8 @@ struct CTab {
   std::vector<BhDev> h_bh;
   int64_t* d_set_off = nullptr;
   double* d_coef = nullptr;
+  int64_t* d_pset_off = nullptr;    // the sets again, padded in the line kernels' LDS order (DevView::pcoef)
+  double* d_pcoef = nullptr;
   BhDev* d_bh = nullptr;
   Dep* d_dep = nullptr;
   double* d_means = nullptr;
@@ -302,6 +304,7 @@ struct gw_ctx {
     if (post_buf) (void)hipFree(post_buf);
     for (auto& kv : ctabs) {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
+      (void)hipFree(kv.second.d_pset_off); (void)hipFree(kv.second.d_pcoef);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
       (void)hipFree(kv.second.d_rchunk); (void)hipFree(kv.second.d_partial);
       (void)hipFree(kv.second.d_agroup);
@@ -332,7 +335,8 @@ struct gw_ctx {
     }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
     for (const Scen& s : sc) v.zc_cls = std::max(v.zc_cls, std::min(s.ncls, kZcMax));
-    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
+    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.pset_off = t->d_pset_off; v.pcoef = t->d_pcoef;
+             v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial;
              v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
     v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size();
@@ -523,6 +527,15 @@ void line_tiles(const Scen& s, int si, int dir, int halo, int lw, std::vector<Li
       t.scen = si; t.line = line; t.kw = k0 - halo; t.ko0 = k0; t.ko1 = k0 + n;
       t.nset = (int)sets.size();
       for (int q = 0; q < 4; ++q) t.set[q] = q < t.nset ? sets[q] : -1;
+      // each lane's slot in set[] (0 for a lane whose set is not among them:
+      // such a lane is past the outputs' halo and feeds no output)
+      t.slot_b0 = t.slot_b1 = 0;
+      for (int kk = 0; kk < lw; ++kk) {
+        const auto it = std::find(sets.begin(), sets.end(), sid_at(t.kw + kk));
+        const uint64_t slot = it == sets.end() ? 0 : (uint64_t)(it - sets.begin());
+        t.slot_b0 |= (slot & 1) << kk;
+        t.slot_b1 |= ((slot >> 1) & 1) << kk;
+      }
       out.push_back(t);
       k0 += n;
     }
@@ -599,6 +612,30 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       eb[l] = L.eb[l];
       fw[l] = (-c * L.eb[l]) * bd[l > 0 ? l - 1 : 0];
       bu[l] = (-c * L.ea[l]) * bd[l];
+    }
+  }
+  // The same values in the order the FP64 line kernels stage them in LDS
+  // (bd, fw, bu, ea, eb), each array padded to its line-length class's
+  // longest line with identity rows (bd 1, the others 0): a tile copies the
+  // arrays it needs with 16-byte LDS DMA loads, no conversion.
+  std::vector<int64_t> pset_off(ctx.sets.size());
+  int64_t ptot = 0;
+  for (size_t q = 0; q < ctx.sets.size(); ++q) {
+    pset_off[q] = ptot;
+    ptot += SET_NARR * (int64_t)kLineClassMax[line_class(ctx.sets[q].N)];
+  }
+  std::vector<double> pcoef((size_t)ptot, 0.0);
+  for (size_t q = 0; q < ctx.sets.size(); ++q) {
+    const int N = ctx.sets[q].N, NM = kLineClassMax[line_class(N)];
+    const double* g = &coef[set_off[q]];
+    double* p = &pcoef[pset_off[q]];
+    for (int l = 0; l < NM; ++l) {
+      const bool ok = l < N;
+      p[0 * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
+      p[1 * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
+      p[2 * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
+      p[3 * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
+      p[4 * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
     }
   }
   // departure tables
@@ -718,6 +755,8 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   try {
     t.d_set_off = dev_upload(set_off);
     t.d_coef = dev_upload(coef);
+    t.d_pset_off = dev_upload(pset_off);
+    t.d_pcoef = dev_upload(pcoef);
     t.d_bh = dev_upload(hb);
     t.d_dep = dev_upload(dep);
     std::vector<double> m(std::max(mean_off, 1), 0.0);
@@ -1539,9 +1578,14 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
   int rc = prepare_steps(*ctx, phi, dt, tab);
   if (rc) return rc;
   if (!tab->adv_err.empty()) return fail(GW_EINVAL, tab->adv_err);
-  hipEvent_t ev[5] = {};
+  // Events 0..1 bracket the first z-step; step n records events 2+4n ..
+  // 2+4n+4 (the YZY's end is the next step's start).  All are recorded
+  // before one synchronisation at the end, so the stream runs the steps back
+  // to back as in a multi-step call.
+  const int nev = 2 + 4 * nsteps + 1;
+  std::vector<hipEvent_t> ev(nev, nullptr);
   for (auto& e : ev)
-    if ((rc = hip_status(hipEventCreate(&e), "event create"))) return rc;
+    if ((rc = hip_status(hipEventCreate(&e), "event create"))) break;
   const double c = dt / 2;
   DevView v = ctx->view(tab);
   auto rec = [&](int q) { return hip_status(hipEventRecord(ev[q], ctx->stream), "event record"); };
@@ -1552,25 +1596,34 @@ int gw_profile_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps
     return r;
   };
   do {
+    if (rc) break;
     void* B = ctx->buf2;
     if ((rc = set_step_ctr(*ctx, t, dt))) break;
     if ((rc = rec(0)) || (rc = fused_zy(*ctx, tab, phi, B, dt)) || (rc = rec(1))) break;
-    if ((rc = hip_status(hipEventSynchronize(ev[1]), "event sync")) || (rc = acc(0, 1, 0))) break;
+    if ((rc = rec(2))) break;
     for (int n = 0; n < nsteps && rc == GW_OK; ++n) {
       const double te = t + (n + 1) * dt;
       const bool last = n + 1 == nsteps;
       void* cur = (n & 1) ? phi : B;
       void* next = last ? phi : ((n & 1) ? B : phi);
-      if ((rc = rec(0)) || (rc = sub_step_advect(*ctx, tab, cur, te)) || (rc = rec(1))) break;
+      const int e0 = 2 + 4 * n;
+      if ((rc = sub_step_advect(*ctx, tab, cur, te)) || (rc = rec(e0 + 1))) break;
       if ((rc = hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT_EXPLICIT, 0, ctx->lw(LINE_IMPLICIT_EXPLICIT, 0), cur, cur, c, 0.0, 0.0, ctx->stream, ctx->lsp()),
-                           "x")) || (rc = rec(2)))
+                           "x")) || (rc = rec(e0 + 2)))
         break;
-      if ((rc = sub_step_advect(*ctx, tab, cur, te + c)) || (rc = rec(3))) break;
+      if ((rc = sub_step_advect(*ctx, tab, cur, te + c)) || (rc = rec(e0 + 3))) break;
       rc = last ? hip_status(launch_line(v, ctx->prec, LINE_IMPLICIT, 1, ctx->lw(LINE_IMPLICIT, 1), next, cur, c, 0.0, 0.0, ctx->stream, ctx->lsp()), "y")
                 : hip_status(launch_line(v, ctx->prec, LINE_YZY, 1, ctx->lw(LINE_YZY, 1), next, cur, c, dt, ctx->c3, ctx->stream, ctx->lsp()), "yzy");
-      if (rc || (rc = rec(4)) || (rc = advance_step_ctr(*ctx))) break;
-      if ((rc = hip_status(hipEventSynchronize(ev[4]), "event sync"))) break;
-      if ((rc = acc(0, 1, 1)) || (rc = acc(1, 2, 2)) || (rc = acc(2, 3, 3)) || (rc = acc(3, 4, last ? 5 : 4))) break;
+      if (rc || (rc = advance_step_ctr(*ctx)) || (rc = rec(e0 + 4))) break;
+    }
+    if (rc || (rc = hip_status(hipEventSynchronize(ev[nev - 1]), "event sync"))) break;
+    if ((rc = acc(0, 1, 0))) break;
+    for (int n = 0; n < nsteps && rc == GW_OK; ++n) {
+      const int e0 = 2 + 4 * n;
+      const bool last = n + 1 == nsteps;
+      if ((rc = acc(e0, e0 + 1, 1)) || (rc = acc(e0 + 1, e0 + 2, 2)) || (rc = acc(e0 + 2, e0 + 3, 3)) ||
+          (rc = acc(e0 + 3, e0 + 4, last ? 5 : 4)))
+        break;
     }
   } while (false);
   for (auto& e : ev)

@@ -70,9 +70,12 @@ struct Dep {
 // j (or y-line i) = `line` in scenario `scen`; lanes with ko0 <= k < ko1 are
 // written.  Its lanes use the nset coefficient sets in `set` (at most 3 in
 // plain tiles, 4 in the z-halo tiles of the fused z-step; unused entries -1).
+// Lane kk's set is set[slot], slot = bit kk of slot_b0 + 2 * bit kk of
+// slot_b1 (computed on the host from the zone and set-id maps).
 struct LineTile {
   int32_t scen, line, kw, ko0, ko1, nset;
   int32_t set[4];
+  uint64_t slot_b0, slot_b1;
 };
 
 // Coefficient set of one 1-D line (length N) for one half-step length c,
