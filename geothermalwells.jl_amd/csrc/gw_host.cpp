@@ -202,8 +202,7 @@ struct CTab {
   std::vector<BhDev> h_bh;
   int64_t* d_set_off = nullptr;
   double* d_coef = nullptr;
-  int64_t* d_pset_off = nullptr;    // the sets again, padded in the line kernels' LDS order (DevView::pcoef)
-  double* d_pcoef = nullptr;
+  double* d_pcoef = nullptr;        // the sets again, padded in the line kernels' LDS order (DevView::pcoef)
   BhDev* d_bh = nullptr;
   Dep* d_dep = nullptr;
   double* d_means = nullptr;
@@ -304,7 +303,7 @@ struct gw_ctx {
     if (post_buf) (void)hipFree(post_buf);
     for (auto& kv : ctabs) {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
-      (void)hipFree(kv.second.d_pset_off); (void)hipFree(kv.second.d_pcoef);
+      (void)hipFree(kv.second.d_pcoef);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
       (void)hipFree(kv.second.d_rchunk); (void)hipFree(kv.second.d_partial);
       (void)hipFree(kv.second.d_agroup);
@@ -335,7 +334,7 @@ struct gw_ctx {
     }
     v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
     for (const Scen& s : sc) v.zc_cls = std::max(v.zc_cls, std::min(s.ncls, kZcMax));
-    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.pset_off = t->d_pset_off; v.pcoef = t->d_pcoef;
+    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.pcoef = t->d_pcoef;
              v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial;
              v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
@@ -493,7 +492,21 @@ void build_line_sets(gw_ctx& ctx, Scen& s, std::unordered_map<std::string, int>&
 // consecutive levels for each output, and a layer boundary a few levels above
 // a borehole bottom (the casing's bottom level below it) puts 4 there.
 
-void line_tiles(const Scen& s, int si, int dir, int halo, int lw, std::vector<LineTile>& out) {
+// Offsets (in doubles) of each set's padded copy in DevView::pcoef: SET_NARR
+// arrays of its line-length class's longest line.  They depend on the sets'
+// lengths only, so every table of the context and the tiles agree on them.
+std::vector<int64_t> padded_set_offsets(const std::vector<LineSet>& sets) {
+  std::vector<int64_t> off(sets.size());
+  int64_t tot = 0;
+  for (size_t q = 0; q < sets.size(); ++q) {
+    off[q] = tot;
+    tot += SET_NARR * (int64_t)kLineClassMax[line_class(sets[q].N)];
+  }
+  return off;
+}
+
+void line_tiles(const Scen& s, int si, int dir, int halo, int lw, const std::vector<int64_t>& pset_off,
+                std::vector<LineTile>& out) {
   const int nlines = dir == 0 ? s.ny : s.nx;
   const std::vector<int32_t>& ids = dir == 0 ? s.xset : s.yset;
   const int width = lw - 2 * halo;
@@ -527,6 +540,11 @@ void line_tiles(const Scen& s, int si, int dir, int halo, int lw, std::vector<Li
       t.scen = si; t.line = line; t.kw = k0 - halo; t.ko0 = k0; t.ko1 = k0 + n;
       t.nset = (int)sets.size();
       for (int q = 0; q < 4; ++q) t.set[q] = q < t.nset ? sets[q] : -1;
+      for (int q = 0; q < 4; ++q) t.poff[q] = q < t.nset ? (uint32_t)pset_off[sets[q]] : 0u;
+      t.N = dir == 0 ? s.nx : s.ny;
+      t.stride = dir == 0 ? s.nz * s.ny : s.nz;
+      t.nz = s.nz;
+      t.base = s.off + (dir == 0 ? (int64_t)s.nz * line : (int64_t)s.nz * s.ny * line) + std::max(t.kw, 0);
       // each lane's slot in set[] (0 for a lane whose set is not among them:
       // such a lane is past the outputs' halo and feeds no output)
       t.slot_b0 = t.slot_b1 = 0;
@@ -618,13 +636,9 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   // (bd, fw, bu, ea, eb), each array padded to its line-length class's
   // longest line with identity rows (bd 1, the others 0): a tile copies the
   // arrays it needs with 16-byte LDS DMA loads, no conversion.
-  std::vector<int64_t> pset_off(ctx.sets.size());
-  int64_t ptot = 0;
-  for (size_t q = 0; q < ctx.sets.size(); ++q) {
-    pset_off[q] = ptot;
-    ptot += SET_NARR * (int64_t)kLineClassMax[line_class(ctx.sets[q].N)];
-  }
-  std::vector<double> pcoef((size_t)ptot, 0.0);
+  const std::vector<int64_t> pset_off = padded_set_offsets(ctx.sets);
+  const int64_t ptot = ctx.sets.empty() ? 0 : pset_off.back() + SET_NARR * (int64_t)kLineClassMax[line_class(ctx.sets.back().N)];
+  std::vector<double> pcoef((size_t)std::max<int64_t>(ptot, 2), 0.0);
   for (size_t q = 0; q < ctx.sets.size(); ++q) {
     const int N = ctx.sets[q].N, NM = kLineClassMax[line_class(N)];
     const double* g = &coef[set_off[q]];
@@ -755,7 +769,6 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   try {
     t.d_set_off = dev_upload(set_off);
     t.d_coef = dev_upload(coef);
-    t.d_pset_off = dev_upload(pset_off);
     t.d_pcoef = dev_upload(pcoef);
     t.d_bh = dev_upload(hb);
     t.d_dep = dev_upload(dep);
@@ -1047,6 +1060,9 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       }
     std::vector<ScenDev> sd;
     std::vector<int64_t> cellp{0}, colp{0};
+    const std::vector<int64_t> pset_off = padded_set_offsets(ctx->sets);
+    if (!pset_off.empty() && pset_off.back() > (int64_t)UINT32_MAX / 2)
+      throw std::runtime_error("too many coefficient sets");
     std::vector<LineTile> xb[2][kLineClasses], yb[2][kLineClasses], hb[2][kLineClasses];
     std::vector<uint16_t> cls;
     std::vector<int32_t> zone, setid;
@@ -1070,9 +1086,9 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       if (qx < 0 || qy < 0) throw std::runtime_error("grid lines longer than the line kernels support");
       for (int f = 0; f < 2; ++f) {
         const int lw = f == 0 ? 64 : 32;
-        line_tiles(s, (int)sd.size() - 1, 0, 0, lw, xb[f][qx]);
-        line_tiles(s, (int)sd.size() - 1, 1, 0, lw, yb[f][qy]);
-        line_tiles(s, (int)sd.size() - 1, 1, 3, lw, hb[f][qy]);
+        line_tiles(s, (int)sd.size() - 1, 0, 0, lw, pset_off, xb[f][qx]);
+        line_tiles(s, (int)sd.size() - 1, 1, 0, lw, pset_off, yb[f][qy]);
+        line_tiles(s, (int)sd.size() - 1, 1, 3, lw, pset_off, hb[f][qy]);
       }
       for (size_t b = 0; b < s.bh.size(); ++b)
         inlet_tab.push_back(s.inlet.kind == GW_INLET_PER_WELL ? s.per_well[b] : 0.0);

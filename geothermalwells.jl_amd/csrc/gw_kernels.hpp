@@ -46,8 +46,8 @@ struct DevView {
   const int64_t* set_off;
   const double* coef;
   // the sets padded in the FP64 line kernels' LDS order: bd, fw, bu, ea, eb,
-  // each kLineClassMax[line_class(N)] long (identity rows past N)
-  const int64_t* pset_off;
+  // each kLineClassMax[line_class(N)] long (identity rows past N), at the
+  // offsets LineTile::poff gives
   const double* pcoef;
   const BhDev* bh;
   int32_t nbh;

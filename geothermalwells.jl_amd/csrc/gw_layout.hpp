@@ -71,11 +71,18 @@ struct Dep {
 // written.  Its lanes use the nset coefficient sets in `set` (at most 3 in
 // plain tiles, 4 in the z-halo tiles of the fused z-step; unused entries -1).
 // Lane kk's set is set[slot], slot = bit kk of slot_b0 + 2 * bit kk of
-// slot_b1 (computed on the host from the zone and set-id maps).
+// slot_b1 (computed on the host from the zone and set-id maps).  N, stride
+// and nz are the line's length, its element stride in the state and the
+// scenario's nz; base is the state offset of (element 0, level max(kw, 0));
+// poff[q] the offset of set[q]'s padded copy in DevView::pcoef: a tile
+// addresses its line and coefficients from its descriptor alone.
 struct LineTile {
   int32_t scen, line, kw, ko0, ko1, nset;
   int32_t set[4];
   uint64_t slot_b0, slot_b1;
+  int32_t N, stride, nz, pad;
+  int64_t base;
+  uint32_t poff[4];
 };
 
 // Coefficient set of one 1-D line (length N) for one half-step length c,

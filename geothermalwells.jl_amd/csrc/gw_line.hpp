@@ -114,6 +114,47 @@ constexpr int line_min_waves() {
   return Cfg::NW <= 4 ? (Cfg::S <= 52 ? 3 : 2) : (Cfg::S <= 32 ? 4 : 2);
 }
 
+// Carries of the segmented recurrences.  Segment g leaves the map
+// v -> m.x + m.y v (its end value from zero carry, and the product of its
+// multipliers) in sh[g][kk]; a lane's carry composes the maps of the
+// segments before it (forward) or after it (backward), reading four of them
+// per LDS round trip.  The loop bounds are the wave's (its last lane's
+// segment forward, its first backward), the lane's own segment masks the rest.
+template <int G, int LW, int H, int B>
+GW_DEV double carry_fwd(const double2* sh, int kk, int seg, int wv) {
+  double C = 0.0;
+  const int gmax = wv * H + H - 1;  // most segments any lane of the wave composes
+#pragma unroll
+  for (int g0 = 0; g0 < G; g0 += B) {
+    if (g0 >= gmax) break;
+    double2 m[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (g0 + i < G) m[i] = sh[(g0 + i) * LW + kk];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (g0 + i < G && g0 + i < seg) C = m[i].x + m[i].y * C;
+  }
+  return C;
+}
+template <int G, int LW, int H, int B>
+GW_DEV double carry_bwd(const double2* sh, int kk, int seg, int wv) {
+  double X = 0.0;
+  const int gmin = wv * H;  // the wave's first segment: every lane composes the segments after its own
+#pragma unroll
+  for (int g0 = G - 1; g0 >= 0; g0 -= B) {
+    if (g0 <= gmin) break;
+    double2 m[B];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (g0 - i >= 0) m[i] = sh[(g0 - i) * LW + kk];
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+      if (g0 - i >= 0 && g0 - i > seg) X = m[i].x + m[i].y * X;
+  }
+  return X;
+}
+
 // Column class of each line element (the last one's past N), loaded into
 // registers and then written to LDS.
 template <class Cfg>
@@ -153,6 +194,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* shA = reinterpret_cast<double*>(smem_raw);  // [G][LW] carries / edges
   double* shB = shA + G * LW;                          // [G][LW]
+  double2* shM = reinterpret_cast<double2*>(shA);      // [G][LW] segment maps of the carry scans (over shA, shB)
   double* shZ = shB + G * LW;                          // [zcls][LW][2]: z coefficients (halo modes)
   // The explicit step and the z-step run in the state type, their LDS
   // coefficients are of type CT (LineCT).
@@ -172,8 +214,10 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   const int kk = lane % LW;    // k offset in the window
   const int hh = lane / LW;    // half-wave (0 when LW = 64)
   const int seg = wv * H + hh;
-  const int N = dir == 0 ? S_.nx : S_.ny;
-  const int nz = S_.nz;
+  // the line's geometry from the tile descriptor (the scenario's fields are
+  // only needed by the z-step staging and the stochastic rounding)
+  const int N = T.N;
+  const int nz = T.nz;
   const int k = T.kw + kk;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
   const int kmin = T.kw < 0 ? 0 : T.kw;
@@ -181,9 +225,9 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   // the upper half-wave, S line elements (below 2^32: nx, ny <= 640,
   // nz <= 4096, S <= 52)
   const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(TS) +
-                      (unsigned)(hh * S) * (unsigned)(dir == 0 ? nz * S_.ny : nz) * (unsigned)sizeof(TS);
-  const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
-  const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
+                      (unsigned)(hh * S) * (unsigned)T.stride * (unsigned)sizeof(TS);
+  const int64_t stride = T.stride;
+  const int64_t base = T.base;
 
   // the coefficient slot of this lane's level (set per tile on the host)
   const int slot = (int)((T.slot_b0 >> kk) & 1u) | ((int)((T.slot_b1 >> kk) & 1u) << 1);
@@ -202,7 +246,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
     // order), addressed before the line's loads go out
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      if (q < nset) pset[q] = v.pcoef + v.pset_off[set_q[q]] + (MT::impl ? 0 : 3 * NM);
+      if (q < nset) pset[q] = v.pcoef + T.poff[q] + (MT::impl ? 0 : 3 * NM);
   } else {
     // coefficient sets of the window's lanes, identity rows beyond N; each
     // set's values are all loaded before any is written (one round trip per set)
@@ -293,7 +337,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   LcgJump sr_j{1u, 0u};
   if constexpr (SR) {
     sr_n = *v.step_ctr;
-    sr_idx = (uint32_t)(S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kc);
+    sr_idx = (uint32_t)(base - kmin + kc);  // the first element's flat index
     constexpr LcgJump js = lcg_jump(S);  // to element l0 = seg * S
     for (int g = 0; g < seg; ++g) sr_j = LcgJump{(sr_j.a * js.a) & kLcgMask, (sr_j.c * js.a + js.c) & kLcgMask};
   }
@@ -360,6 +404,13 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   constexpr int KC = ((MT::halo && S > 40 && sizeof(TS) == 8) || (sizeof(TS) == 4 && Cfg::NW <= 4 && S > 32 && S <= 40) ||
                       (SR && MT::halo && Cfg::NW == 8 && S == 40) || S % 4 != 0) ? 2 : 4;
   static_assert(S % KC == 0, "segment length must be a multiple of the chunk");
+  // The carry scans of the FP64 sixteen-segment fused z-step tiles read four
+  // segment maps per LDS round trip (interleaved {value, product} pairs:
+  // 1.5% off the 401-point lines' fused kernel).  The other shapes read one
+  // map per step from two arrays
+  // (the batched form's extra registers cost several of them a wave per
+  // SIMD, and it made the 401-point implicit-y tiles 4.6% slower).
+  constexpr bool kScan4 = sizeof(TS) == 8 && G >= 16 && MT::halo;
   if constexpr (MT::impl && sizeof(TS) == 4) {
     // Forward elimination y_l = b_l - fw_l y_{l-1}, then back substitution
     // x_l = bd_l y_l - bu_l x_{l+1}, each in two passes over the segment:
@@ -496,11 +547,17 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
         d[r0 + q] = y;
       }
     }
-    shA[seg * LW + kk] = y;
-    shB[seg * LW + kk] = P;
-    __syncthreads();
     double C = 0.0;
-    for (int g = 0; g < seg; ++g) C = shA[g * LW + kk] + shB[g * LW + kk] * C;
+    if constexpr (kScan4) {
+      shM[seg * LW + kk] = make_double2(y, P);
+      __syncthreads();
+      C = carry_fwd<G, LW, H, 4>(shM, kk, seg, wv);
+    } else {
+      shA[seg * LW + kk] = y;
+      shB[seg * LW + kk] = P;
+      __syncthreads();
+      for (int g = 0; g < seg; ++g) C = shA[g * LW + kk] + shB[g * LW + kk] * C;
+    }
     __syncthreads();
     P = 1.0;
 #pragma unroll
@@ -549,11 +606,17 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
         d[r] = x;
       }
     }
-    shA[seg * LW + kk] = x;
-    shB[seg * LW + kk] = Q;
-    __syncthreads();
     double X = 0.0;
-    for (int g = G - 1; g > seg; --g) X = shA[g * LW + kk] + shB[g * LW + kk] * X;
+    if constexpr (kScan4) {
+      shM[seg * LW + kk] = make_double2(x, Q);
+      __syncthreads();
+      X = carry_bwd<G, LW, H, 4>(shM, kk, seg, wv);
+    } else {
+      shA[seg * LW + kk] = x;
+      shB[seg * LW + kk] = Q;
+      __syncthreads();
+      for (int g = G - 1; g > seg; --g) X = shA[g * LW + kk] + shB[g * LW + kk] * X;
+    }
     __syncthreads();
     Q = 1.0;
 #pragma unroll
