@@ -42,6 +42,12 @@ GW_DEV void lds_dma16(const void* g, void* lds_wave) {
   __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave, 16, 0, 0);
 }
 
+// The same with 4 bytes per lane (global_load_lds_dword): lane i to
+// lds_wave + 4 i.
+GW_DEV void lds_dma4(const void* g, void* lds_wave) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave, 4, 0, 0);
+}
+
 // A wave-uniform element offset pinned in SGPRs: base + offset + lane byte
 // offset then uses the scalar-base + 32-bit vector-offset addressing form.
 GW_DEV int64_t soff(int64_t x) {

@@ -202,7 +202,7 @@ struct CTab {
   std::vector<BhDev> h_bh;
   int64_t* d_set_off = nullptr;
   double* d_coef = nullptr;
-  double* d_pcoef = nullptr;        // the sets again, padded in the line kernels' LDS order (DevView::pcoef)
+  double* d_pcoef[2] = {};          // the sets again, padded in the line kernels' LDS layout (DevView::pcoef)
   BhDev* d_bh = nullptr;
   Dep* d_dep = nullptr;
   double* d_means = nullptr;
@@ -247,6 +247,7 @@ struct gw_ctx {
   uint16_t* d_cls = nullptr;
   int32_t* d_zone = nullptr;
   double* d_zcoef = nullptr;
+  float* d_zcoef32 = nullptr;       // the same rounded to float (FP32 states)
   int32_t* d_setid = nullptr;
   PipeCol* d_pcol = nullptr;
   double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
@@ -303,7 +304,7 @@ struct gw_ctx {
     if (post_buf) (void)hipFree(post_buf);
     for (auto& kv : ctabs) {
       (void)hipFree(kv.second.d_set_off); (void)hipFree(kv.second.d_coef); (void)hipFree(kv.second.d_bh);
-      (void)hipFree(kv.second.d_pcoef);
+      (void)hipFree(kv.second.d_pcoef[0]); (void)hipFree(kv.second.d_pcoef[1]);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
       (void)hipFree(kv.second.d_rchunk); (void)hipFree(kv.second.d_partial);
       (void)hipFree(kv.second.d_agroup);
@@ -332,9 +333,9 @@ struct gw_ctx {
         }
       }
     }
-    v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.setid = d_setid;
+    v.cls = d_cls; v.zone = d_zone; v.zcoef = d_zcoef; v.zcoef32 = d_zcoef32; v.setid = d_setid;
     for (const Scen& s : sc) v.zc_cls = std::max(v.zc_cls, std::min(s.ncls, kZcMax));
-    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.pcoef = t->d_pcoef;
+    if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.pcoef[0] = t->d_pcoef[0]; v.pcoef[1] = t->d_pcoef[1] ? t->d_pcoef[1] : t->d_pcoef[0];
              v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial;
              v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
@@ -632,24 +633,39 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
       bu[l] = (-c * L.ea[l]) * bd[l];
     }
   }
-  // The same values in the order the FP64 line kernels stage them in LDS
-  // (bd, fw, bu, ea, eb), each array padded to its line-length class's
-  // longest line with identity rows (bd 1, the others 0): a tile copies the
-  // arrays it needs with 16-byte LDS DMA loads, no conversion.
+  // The same values in the layout the line kernels keep in an LDS slot (bd,
+  // fw, bu, then the explicit coefficients), each array padded to its
+  // line-length class's longest line with identity rows (bd 1, the others 0):
+  // a tile copies the arrays it needs with 16-byte LDS DMA loads.  FP64
+  // states: ea, eb.  FP32 states: the explicit step's products c ea, c eb,
+  // as doubles for the 32-lane family and rounded to float (an array of NM
+  // floats at the start of each NM-double array) for the 64-lane family.
   const std::vector<int64_t> pset_off = padded_set_offsets(ctx.sets);
   const int64_t ptot = ctx.sets.empty() ? 0 : pset_off.back() + SET_NARR * (int64_t)kLineClassMax[line_class(ctx.sets.back().N)];
-  std::vector<double> pcoef((size_t)std::max<int64_t>(ptot, 2), 0.0);
-  for (size_t q = 0; q < ctx.sets.size(); ++q) {
-    const int N = ctx.sets[q].N, NM = kLineClassMax[line_class(N)];
-    const double* g = &coef[set_off[q]];
-    double* p = &pcoef[pset_off[q]];
-    for (int l = 0; l < NM; ++l) {
-      const bool ok = l < N;
-      p[0 * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
-      p[1 * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
-      p[2 * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
-      p[3 * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
-      p[4 * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
+  const bool f32 = ctx.prec != GW_F64;
+  std::vector<double> pcoef[2];
+  for (int f = 0; f < (f32 ? 2 : 1); ++f) {
+    pcoef[f].assign((size_t)std::max<int64_t>(ptot, 2), 0.0);
+    const double cs = f32 ? c : 1.0;
+    for (size_t q = 0; q < ctx.sets.size(); ++q) {
+      const int N = ctx.sets[q].N, NM = kLineClassMax[line_class(N)];
+      const double* g = &coef[set_off[q]];
+      double* p = &pcoef[f][pset_off[q]];
+      for (int l = 0; l < NM; ++l) {
+        const bool ok = l < N;
+        p[0 * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
+        p[1 * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
+        p[2 * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
+        const double ea = ok ? cs * g[SET_EA * N + l] : 0.0;
+        const double eb = ok ? cs * g[SET_EB * N + l] : 0.0;
+        if (f32 && f == 0) {
+          reinterpret_cast<float*>(p + 3 * NM)[l] = (float)ea;
+          reinterpret_cast<float*>(p + 4 * NM)[l] = (float)eb;
+        } else {
+          p[3 * NM + l] = ea;
+          p[4 * NM + l] = eb;
+        }
+      }
     }
   }
   // departure tables
@@ -769,7 +785,8 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   try {
     t.d_set_off = dev_upload(set_off);
     t.d_coef = dev_upload(coef);
-    t.d_pcoef = dev_upload(pcoef);
+    t.d_pcoef[0] = dev_upload(pcoef[0]);
+    t.d_pcoef[1] = f32 ? dev_upload(pcoef[1]) : nullptr;
     t.d_bh = dev_upload(hb);
     t.d_dep = dev_upload(dep);
     std::vector<double> m(std::max(mean_off, 1), 0.0);
@@ -1133,6 +1150,11 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     ctx->d_cls = ctx->up(cls);
     ctx->d_zone = ctx->up(zone);
     ctx->d_zcoef = ctx->up(zcoef);
+    if (ctx->prec != GW_F64) {  // the FP32 64-lane kernels' z coefficients
+      std::vector<float> z32(zcoef.begin(), zcoef.end());
+      if (z32.empty()) z32.push_back(0.f);
+      ctx->d_zcoef32 = ctx->up(z32);
+    }
     ctx->d_setid = ctx->up(setid);
     ctx->d_pcol = ctx->up(ctx->pcol);
     std::vector<double> zeros(std::max(ctx->nbh_total, 1), 0.0);

@@ -41,14 +41,17 @@ struct DevView {
   const uint16_t* cls;
   const int32_t* zone;
   const double* zcoef;          // pairs {ea, eb}
+  const float* zcoef32;         // the same rounded to float (FP32 states only)
   const int32_t* setid;
   // dt-dependent
   const int64_t* set_off;
   const double* coef;
-  // the sets padded in the FP64 line kernels' LDS order: bd, fw, bu, ea, eb,
-  // each kLineClassMax[line_class(N)] long (identity rows past N), at the
-  // offsets LineTile::poff gives
-  const double* pcoef;
+  // the sets padded in the line kernels' LDS slot layout, per k-window family
+  // (index 0: 64-lane, 1: 32-lane): bd, fw, bu, then the explicit
+  // coefficients (FP64 states ea, eb; FP32 states c ea, c eb, as float
+  // arrays in the 64-lane family), each kLineClassMax[line_class(N)] long
+  // with identity rows past N, at the offsets LineTile::poff gives
+  const double* pcoef[2];
   const BhDev* bh;
   int32_t nbh;
   const PipeCol* pcol;

@@ -232,97 +232,55 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   // the coefficient slot of this lane's level (set per tile on the host)
   const int slot = (int)((T.slot_b0 >> kk) & 1u) | ((int)((T.slot_b1 >> kk) & 1u) << 1);
   constexpr int NL = (NM + Cfg::THREADS - 1) / Cfg::THREADS;  // staged elements per thread and array
-  // FP64 states issue the line's loads first and then copy the coefficient
-  // sets and z coefficients into LDS by LDS DMA (no registers and no wait:
-  // the barrier after the line's loads waits for them), so a tile's loads go
-  // out after its two scalar round trips; FP32 states stage through
-  // registers (converting to the coefficient type) before the line's loads.
-  constexpr bool kDma = sizeof(TS) == 8;
-  const int nset = T.nset;
-  const int set_q[4] = {T.set[0], T.set[1], T.set[2], T.set[3]};
-  const double* pset[4] = {nullptr, nullptr, nullptr, nullptr};
-  if constexpr (kDma) {
-    // the sets' padded copies (bd, fw, bu, ea, eb, NM each: the LDS slot's
-    // order), addressed before the line's loads go out
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (q < nset) pset[q] = v.pcoef + T.poff[q] + (MT::impl ? 0 : 3 * NM);
-  } else {
-    // coefficient sets of the window's lanes, identity rows beyond N; each
-    // set's values are all loaded before any is written (one round trip per set)
-    const double cs_expl = sizeof(TS) == 4 ? c : 1.0;
-    for (int q = 0; q < nset; ++q) {
-      const int sq = q == 0 ? set_q[0] : (q == 1 ? set_q[1] : (q == 2 ? set_q[2] : set_q[3]));
-      const double* g = v.coef + v.set_off[sq];
+  // A tile's loads go out right after its descriptor's scalar round trip:
+  // the coefficient sets and z coefficients are copied into LDS by LDS DMA
+  // (no registers, no wait: the barrier after the line's loads waits for
+  // them), from copies laid out as the LDS holds them; the column classes
+  // go through registers, written to LDS after the line's loads are issued.
+  {
+    // the sets' padded copies (bd, fw, bu, explicit coefficients, NM each:
+    // the LDS slot's layout), 16 bytes per lane and DMA load
+    constexpr int NCH = MT::NA * NM / 2;  // 16-byte chunks of a slot
+    const double* pc = v.pcoef[FAM] + (MT::impl ? 0 : 3 * NM);
+    for (int q = 0; q < T.nset; ++q) {
+      const double* pg = pc + (q == 0 ? T.poff[0] : (q == 1 ? T.poff[1] : (q == 2 ? T.poff[2] : T.poff[3])));
       double* dst = shC + q * MT::NA * NM;
-      double cv[NL][5];
 #pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        const int l = tid + i * Cfg::THREADS;
-        const bool ok = l < N;
-        if (MT::impl) {
-          cv[i][0] = ok ? g[SET_BD * N + l] : 1.0;
-          cv[i][1] = ok ? g[SET_FW * N + l] : 0.0;
-          cv[i][2] = ok ? g[SET_BU * N + l] : 0.0;
-        }
-        if (MT::expl) {
-          // FP32 lines stage the explicit step's products c ea, c eb
-          cv[i][3] = ok ? cs_expl * g[SET_EA * N + l] : 0.0;
-          cv[i][4] = ok ? cs_expl * g[SET_EB * N + l] : 0.0;
+      for (int i = 0; i < (NCH + Cfg::THREADS - 1) / Cfg::THREADS; ++i) {
+        const int cw = i * Cfg::THREADS + wv * 64;  // the wave's first chunk
+        if (cw < NCH) {
+          const int ch = cw + lane;
+          if (ch < NCH) lds_dma16(pg + 2 * ch, dst + 2 * cw);
         }
       }
-#pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        const int l = tid + i * Cfg::THREADS;
-        if (l < NM) {
-          if (MT::impl) {
-            dst[MT::A_BD * NM + l] = cv[i][0];
-            dst[MT::A_FW * NM + l] = cv[i][1];
-            dst[MT::A_BU * NM + l] = cv[i][2];
-          }
-          if (MT::expl) {
-            if constexpr (std::is_same<CT, double>::value) {
-              dst[MT::A_EA * NM + l] = cv[i][3];
-              dst[MT::A_EB * NM + l] = cv[i][4];
-            } else {
-              reinterpret_cast<CT*>(dst + MT::A_EA * NM)[l] = (CT)cv[i][3];
-              reinterpret_cast<CT*>(dst + MT::A_EB * NM)[l] = (CT)cv[i][4];
-            }
-          }
-        }
-      }
-    }
-    if (MT::halo) {
-      const double* zc = v.zcoef + 2 * S_.o_zcoef;
-      const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
-      // classes in batches of four per wave: loads first, then the writes
-      for (int q0 = wv; q0 < ncls; q0 += 4 * Cfg::NW) {
-        double z0[4], z1[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int q = q0 + i * Cfg::NW;
-          z0[i] = q < ncls ? zc[2 * ((int64_t)q * nz + kc) + hh] : 0.0;
-          if (H == 1) z1[i] = q < ncls ? zc[2 * ((int64_t)q * nz + kc) + 1] : 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int q = q0 + i * Cfg::NW;
-          if (q < ncls) {
-            if constexpr (std::is_same<CT, double>::value) {
-              shZ[(q * LW + kk) * 2 + hh] = z0[i];
-              if (H == 1) shZ[(q * LW + kk) * 2 + 1] = z1[i];
-            } else {
-              shZt[(q * LW + kk) * 2 + hh] = (CT)z0[i];
-              if (H == 1) shZt[(q * LW + kk) * 2 + 1] = (CT)z1[i];
-            }
-          }
-        }
-      }
-      uint16_t kv[NL];
-      load_line_classes<Cfg>(v.cls + S_.o_cls + (int64_t)S_.ny * T.line, N, tid, kv);
-      store_line_classes<Cfg>(kv, shK, tid);
     }
   }
+  if (MT::halo) {
+    const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
+    if constexpr (std::is_same<CT, double>::value) {
+      // {ea, eb} of level kc: one class per half-wave (LW = 32) or wave
+      // (LW = 64), lane (hh, kk) copies class q0 + hh
+      const double* zc = v.zcoef + 2 * S_.o_zcoef;
+      for (int q0 = wv * H; q0 < ncls; q0 += Cfg::NW * H) {
+        const int q = q0 + hh;
+        if (q < ncls) lds_dma16(zc + 2 * ((int64_t)q * nz + kc), shZ + q0 * LW * 2);
+      }
+    } else {
+      // float pairs (64-lane FP32 tiles): a class is 64 lanes x 8 bytes, two
+      // 4-byte DMA loads per class, lane i copying component i & 1 of the
+      // pair of level kw + i / 2 (+ 32 in the second load)
+      const float* zc = v.zcoef32 + 2 * S_.o_zcoef;
+      for (int q = wv; q < ncls; q += Cfg::NW)
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          const int kq = T.kw + half * 32 + (lane >> 1);
+          const int kqc = kq < 0 ? 0 : (kq >= nz ? nz - 1 : kq);
+          lds_dma4(zc + 2 * ((int64_t)q * nz + kqc) + (lane & 1), shZt + (q * LW + half * 32) * 2);
+        }
+    }
+  }
+  uint16_t kv[NL];
+  if (MT::halo) load_line_classes<Cfg>(v.cls + S_.o_cls + (int64_t)S_.ny * T.line, N, tid, kv);
 
   RT d[S];
   const int l0 = seg * S;
@@ -344,35 +302,6 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
   auto sr_first = [&](uint32_t n, int p) { return lcg_apply(sr_j, sr_stream(sr_idx, n, (uint32_t)(dir * 4 + p))); };
   // the z-step and explicit y of YZY belong to the next step
   const uint32_t sr_n2 = MODE == LINE_YZY ? sr_n + 1u : sr_n;
-  if constexpr (kDma) {
-    constexpr int NCH = MT::NA * NM / 2;  // 16-byte chunks of a slot
-    for (int q = 0; q < nset; ++q) {
-      const double* pg = q == 0 ? pset[0] : (q == 1 ? pset[1] : (q == 2 ? pset[2] : pset[3]));
-      double* dst = shC + q * MT::NA * NM;
-#pragma unroll
-      for (int i = 0; i < (NCH + Cfg::THREADS - 1) / Cfg::THREADS; ++i) {
-        const int cw = i * Cfg::THREADS + wv * 64;  // the wave's first chunk
-        if (cw < NCH) {
-          const int ch = cw + lane;
-          if (ch < NCH) lds_dma16(pg + 2 * ch, dst + 2 * cw);
-        }
-      }
-    }
-    if (MT::halo) {
-      // z coefficients: one class per half-wave (LW = 32) or wave (LW = 64),
-      // lane (hh, kk) copies {ea, eb} of class q0 + hh at level kc
-      const double* zc = v.zcoef + 2 * S_.o_zcoef;
-      const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
-      for (int q0 = wv * H; q0 < ncls; q0 += Cfg::NW * H) {
-        const int q = q0 + hh;
-        if (q < ncls) lds_dma16(zc + 2 * ((int64_t)q * nz + kc), shZ + q0 * LW * 2);
-      }
-    }
-  }
-  // FP64 halo modes: the line elements' column classes are loaded here and
-  // written to LDS after the line's loads have gone out
-  uint16_t kv[NL];
-  if (kDma && MT::halo) load_line_classes<Cfg>(v.cls + S_.o_cls + (int64_t)S_.ny * T.line, N, tid, kv);
   const int64_t ustride = uniform64(stride);
   const int64_t useg = uniform64(base + (int64_t)(wv * H * S) * stride);
   // element r of the lane's segment: field base + an element offset kept in
@@ -385,7 +314,7 @@ __global__ __launch_bounds__(Cfg::THREADS, (line_min_waves<Cfg, TS, MODE, SR>())
 #pragma unroll
     for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? (RT)*lane_ptr(in + soff(useg + (int64_t)r * ustride), kb) : (RT)0;
   }
-  if (kDma && MT::halo) store_line_classes<Cfg>(kv, shK, tid);
+  if (MT::halo) store_line_classes<Cfg>(kv, shK, tid);
   __syncthreads();
 
   const double* cs = shC + slot * MT::NA * NM;
