@@ -13,6 +13,7 @@ GW_OK, GW_EINVAL, GW_EHIP, GW_ENOMEM, GW_ESTATE = 0, -1, -2, -3, -4
 GW_F64, GW_F32_STATE, GW_F32_STATE_SR = 0, 1, 2
 GW_FLAG_LINE_W64, GW_FLAG_LINE_W32, GW_FLAG_LINE_TWO_TILES, GW_FLAG_HALO_WHOLE = 1, 2, 4, 8
 GW_FLAG_LINE_CONCURRENT = 16
+GW_FLAG_STEPS_GRAPH, GW_FLAG_STEPS_DIRECT = 32, 64
 GW_INLET_CONSTANT, GW_INLET_HEAT_EXCHANGER, GW_INLET_PER_WELL, GW_INLET_HOST = 0, 1, 2, 3
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -95,6 +96,7 @@ SIGNATURES = {
     "gw_eigen_estimate": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_int32]),
     "gw_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "gw_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "gw_steps_mode": (ctypes.c_int, [ctypes.c_void_p]),
     "gw_line_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int32, ctypes.c_int32]),
     "gw_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,

@@ -97,7 +97,8 @@ class Cache:
     advection sub-step."""
 
     def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                 line_window=None, line_two_tiles=False, line_halo_whole=False, line_concurrent=False):
+                 line_window=None, line_two_tiles=False, line_halo_whole=False, line_concurrent=False,
+                 steps_launch=None):
         self._scen = scenarios
         self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
         if self.backend.type != "cuda":
@@ -124,6 +125,9 @@ class Cache:
                       (_abi.GW_FLAG_LINE_TWO_TILES if line_two_tiles else 0) |
                       (_abi.GW_FLAG_HALO_WHOLE if line_halo_whole else 0) |
                       (_abi.GW_FLAG_LINE_CONCURRENT if line_concurrent else 0))
+        if steps_launch not in (None, "graph", "direct"):
+            raise ValueError("steps_launch is None (timed on the first long call), 'graph' or 'direct'")
+        prob.flags |= {None: 0, "graph": _abi.GW_FLAG_STEPS_GRAPH, "direct": _abi.GW_FLAG_STEPS_DIRECT}[steps_launch]
         h = ctypes.c_void_p()
         _abi.check(lib.gw_create(ctypes.byref(prob), ctypes.byref(h)))
         self._ctx = h
@@ -213,30 +217,41 @@ class Cache:
     def sync(self):
         _abi.check(self._lib.gw_sync(self._ctx))
 
+    @property
+    def steps_launch(self):
+        """How multi-step calls run their steps: None (not timed yet),
+        "graph" or "direct"."""
+        m = self._lib.gw_steps_mode(self._ctx)
+        if m < 0:
+            _abi.check(m)
+        return {0: None, 1: "graph", 2: "direct"}[m]
+
 
 def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inlet_model, precision="f64",
                  stream=None, rock2_c3=DEFAULT_ROCK2_C3, line_window=None, line_two_tiles=False,
-                 line_halo_whole=False):
+                 line_halo_whole=False, steps_launch=None):
     """create_cache (src/cache.jl:55-116).  `precision`: "f64" (the
     reference's), "f32" or "f32sr" (FP32 state, see PRECISIONS).  `line_window` forces the k-window
     width (64 or 32 lanes) of the line kernels, and with it `line_two_tiles`
     (LDS padded to two tiles per CU) and `line_halo_whole` (fused z-step tiles
     of a class in one launch); None lets the first multi-step call time the
-    variants and keep the fastest.  Results do not depend on these."""
+    variants and keep the fastest.  `steps_launch` ("graph" or "direct")
+    forces how multi-step calls run their steps (None: timed on the first
+    call of 8 or more steps).  Results do not depend on these."""
     return Cache([_Scenario(gridx, gridy, gridz, materials, boreholes, inlet_model)], backend=backend,
                  precision=precision, stream=stream, rock2_c3=rock2_c3, line_window=line_window,
-                 line_two_tiles=line_two_tiles, line_halo_whole=line_halo_whole)
+                 line_two_tiles=line_two_tiles, line_halo_whole=line_halo_whole, steps_launch=steps_launch)
 
 
 def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                       line_window=None, line_concurrent=False):
+                       line_window=None, line_concurrent=False, steps_launch=None):
     """Many independent scenarios (each a dict with the create_cache keywords
     gridx, gridy, gridz, materials, boreholes, inlet_model) advanced together:
     every kernel launch covers all of them."""
     sc = [_Scenario(d["gridx"], d["gridy"], d["gridz"], d["materials"], d["boreholes"], d["inlet_model"])
           for d in scenarios]
     return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3,
-                 line_window=line_window, line_concurrent=line_concurrent)
+                 line_window=line_window, line_concurrent=line_concurrent, steps_launch=steps_launch)
 
 
 # ---------------------------------------------------------------------------

@@ -228,6 +228,9 @@ T* dev_upload(const std::vector<T>& h) {
 
 }  // namespace
 
+// gw_steps launch modes: not yet timed, graph replays, direct launches.
+enum { kStepsUndecided = 0, kStepsGraph = 1, kStepsEager = 2 };
+
 struct gw_ctx {
   int prec = GW_F64;
   hipStream_t stream = nullptr;
@@ -264,6 +267,7 @@ struct gw_ctx {
   int lw_sel[5][2] = {};
   int force_lw = 0;
   bool tuned = false;
+  int steps_mode = kStepsUndecided;     // gw_steps: graph replays or direct launches (kSteps*)
   int adj_lw[2] = {0, 0};  // k-window width of the adjoint line modes ADJ_Y, ADJ_X (0: not yet timed)
   int lw(int mode, int dir) const { return lw_sel[mode][dir]; }
   bool fused_ok = false;  // fused z-step line kernels usable (few enough column classes)
@@ -969,9 +973,12 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     return fail(GW_EINVAL, "unknown precision");
   const int32_t width = p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32);
   const int32_t variant = p->flags & (GW_FLAG_LINE_TWO_TILES | GW_FLAG_HALO_WHOLE | GW_FLAG_LINE_CONCURRENT);
-  if ((p->flags & ~(width | variant)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) || (variant && !width))
+  const int32_t steps = p->flags & (GW_FLAG_STEPS_GRAPH | GW_FLAG_STEPS_DIRECT);
+  if ((p->flags & ~(width | variant | steps)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) ||
+      (variant && !width) || steps == (GW_FLAG_STEPS_GRAPH | GW_FLAG_STEPS_DIRECT))
     return fail(GW_EINVAL, "unknown or conflicting flags (the launch variants need a forced width)");
   std::unique_ptr<gw_ctx> ctx(new gw_ctx());
+  ctx->steps_mode = steps == GW_FLAG_STEPS_GRAPH ? kStepsGraph : (steps == GW_FLAG_STEPS_DIRECT ? kStepsEager : kStepsUndecided);
   ctx->prec = p->precision;
   ctx->force_lw = width == GW_FLAG_LINE_W64 ? 64 : (width == GW_FLAG_LINE_W32 ? 32 : 0);
   if (ctx->force_lw) {
@@ -1378,6 +1385,14 @@ int gw_step(gw_ctx* ctx, void* phi, double t, double dt) {
   return gw_steps(ctx, phi, t, dt, 1);
 }
 
+// Steps 2 .. nsteps-1 of a gw_steps call replay one captured graph per step
+// or launch the same kernels directly (identical results).  Unless a flag
+// forces one, the first call of at least 2 kStepTune + 2 steps times
+// kStepTune steps of each, after one of each, and keeps the faster: on large
+// fields the replays measured 1-7% slower per step than direct launches on
+// MI355X (ROCm 7), on small fields the launches' host cost can decide.
+constexpr int kStepTune = 3;
+
 int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
   if (!ctx || !phi) return fail(GW_EINVAL, "null argument");
   if (!(dt > 0) || nsteps < 0) return fail(GW_EINVAL, "dt must be positive and nsteps >= 0");
@@ -1399,8 +1414,9 @@ int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
   rc = fused_zy(*ctx, tab, phi, B, dt);
   if (rc) return rc;
   // step k (1-based) works on B when k is odd, on phi when k is even
-  const bool use_graph = !ctx->host_inlet && nsteps > 2;
-  if (use_graph && (!ctx->graph_exec || ctx->graph_phi != phi || ctx->graph_dt != dt)) {
+  const bool graph_ok = !ctx->host_inlet && nsteps > 2 && ctx->steps_mode != kStepsEager;
+  const bool tune = graph_ok && ctx->steps_mode == kStepsUndecided && nsteps >= 2 * kStepTune + 2;
+  if (graph_ok && (!ctx->graph_exec || ctx->graph_phi != phi || ctx->graph_dt != dt)) {
     if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
     if (ctx->graph_exec2) { (void)hipGraphExecDestroy(ctx->graph_exec2); ctx->graph_exec2 = nullptr; }
     if ((rc = capture_body(*ctx, tab, B, phi, t + dt, dt, &ctx->graph_exec2))) return rc;  // B -> phi
@@ -1408,16 +1424,35 @@ int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
     ctx->graph_phi = phi;
     ctx->graph_dt = dt;
   }
-  for (int k = 1; k < nsteps; ++k) {
+  auto step = [&](int k, bool graph) -> int {
     const bool odd = (k & 1) != 0;
-    void* cur = odd ? B : phi;
-    void* next = odd ? phi : B;
-    if (use_graph)
-      rc = hip_status(hipGraphLaunch(odd ? ctx->graph_exec2 : ctx->graph_exec, ctx->stream), "graph launch");
-    else
-      rc = fused_rest(*ctx, tab, cur, next, t + k * dt, dt, false);
+    if (graph) return hip_status(hipGraphLaunch(odd ? ctx->graph_exec2 : ctx->graph_exec, ctx->stream), "graph launch");
+    return fused_rest(*ctx, tab, odd ? B : phi, odd ? phi : B, t + k * dt, dt, false);
+  };
+  int k = 1;
+  if (tune) {
+    hipEvent_t ev[3] = {};
+    for (auto& e : ev)
+      if (!rc) rc = hip_status(hipEventCreate(&e), "event");
+    if (!rc) rc = step(k++, false);
+    if (!rc) rc = step(k++, true);
+    if (!rc) rc = hip_status(hipEventRecord(ev[0], ctx->stream), "event");
+    for (int q = 0; q < kStepTune && !rc; ++q) rc = step(k++, false);
+    if (!rc) rc = hip_status(hipEventRecord(ev[1], ctx->stream), "event");
+    for (int q = 0; q < kStepTune && !rc; ++q) rc = step(k++, true);
+    if (!rc) rc = hip_status(hipEventRecord(ev[2], ctx->stream), "event");
+    if (!rc) rc = hip_status(hipEventSynchronize(ev[2]), "event sync");
+    float direct = 0.f, graph = 0.f;
+    if (!rc) rc = hip_status(hipEventElapsedTime(&direct, ev[0], ev[1]), "event time");
+    if (!rc) rc = hip_status(hipEventElapsedTime(&graph, ev[1], ev[2]), "event time");
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (rc) return rc;
+    ctx->steps_mode = graph < direct ? kStepsGraph : kStepsEager;
   }
+  const bool use_graph = graph_ok && ctx->steps_mode != kStepsEager;
+  for (; k < nsteps; ++k)
+    if ((rc = step(k, use_graph))) return rc;
   void* cur = (nsteps & 1) ? B : phi;
   return fused_rest(*ctx, tab, cur, phi, t + nsteps * dt, dt, true);
 }
@@ -1602,6 +1637,11 @@ int gw_extract_x_profile(const void* T, int32_t precision, int32_t nx, int32_t n
 int gw_sync(gw_ctx* ctx) {
   if (!ctx) return fail(GW_EINVAL, "null argument");
   return hip_status(hipStreamSynchronize(ctx->stream), "sync");
+}
+
+int gw_steps_mode(gw_ctx* ctx) {
+  if (!ctx) return fail(GW_EINVAL, "null argument");
+  return ctx->steps_mode;
 }
 
 void* gw_stream(gw_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }

@@ -54,10 +54,13 @@ enum { GW_F64 = 0, GW_F32_STATE = 1, GW_F32_STATE_SR = 2 };
  * tile's LDS so at most two share a CU, and GW_FLAG_HALO_WHOLE launches the
  * fused z-step tiles of a line class as one range (default: split by their
  * coefficient-set count); GW_FLAG_LINE_CONCURRENT runs the launches of the
- * line-length classes of a batch concurrently on side streams.  Results do not
- * depend on these choices. */
+ * line-length classes of a batch concurrently on side streams.  gw_steps runs
+ * the steps of a call by replaying one captured hipGraph per step or by
+ * launching the same kernels directly; the first call of 8 or more steps times
+ * both and keeps the faster, GW_FLAG_STEPS_GRAPH / _DIRECT force one.  Results
+ * do not depend on these choices. */
 enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2, GW_FLAG_LINE_TWO_TILES = 4, GW_FLAG_HALO_WHOLE = 8,
-       GW_FLAG_LINE_CONCURRENT = 16 };
+       GW_FLAG_LINE_CONCURRENT = 16, GW_FLAG_STEPS_GRAPH = 32, GW_FLAG_STEPS_DIRECT = 64 };
 
 /* Inlet models (src/physics.jl:283-318). */
 enum {
@@ -159,8 +162,9 @@ int gw_step(gw_ctx *ctx, void *phi, double t, double dt);
 /* nsteps full steps from t as the fused step pipeline (two full-field sweeps
  * per step: implicit+explicit x, and implicit y fused with the next step's
  * z-step and explicit y), with the arithmetic of gw_zstep + gw_adi_adv.
- * Replays one captured hipGraph per step when the inlet models allow it (every
- * kind but GW_INLET_HOST).  The first call times the line kernels' two
+ * Steps 2 .. nsteps-1 replay one captured hipGraph per step or launch the
+ * same kernels directly (graphs need inlet models other than GW_INLET_HOST);
+ * see GW_FLAG_STEPS_GRAPH.  The first call times the line kernels' two
  * k-window widths (unless gw_problem.flags forces one); phi is only read by
  * that.  The result is in phi. */
 int gw_steps(gw_ctx *ctx, void *phi, double t, double dt, int32_t nsteps);
@@ -266,6 +270,11 @@ int gw_sync(gw_ctx *ctx);
 
 /* The hipStream_t the context launches on (for callers that record events). */
 void *gw_stream(gw_ctx *ctx);
+
+/* How gw_steps runs a call's steps: 0 not decided yet (graph replays until a
+ * call of 8 or more steps has timed both), 1 graph replays, 2 direct launches;
+ * < 0 on error. */
+int gw_steps_mode(gw_ctx *ctx);
 
 /* Measurement: nsteps full steps (no graph) with HIP events between the
  * kernels of the fused step pipeline, on the context's stream.  ms[] adds up:
