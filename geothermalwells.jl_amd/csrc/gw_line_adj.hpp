@@ -61,50 +61,63 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
   const int kk = lane % LW;
   const int hh = lane / LW;
   const int seg = wv * H + hh;
-  const int N = dir == 0 ? S_.nx : S_.ny;
-  const int nz = S_.nz;
+  const int N = T.N;
+  const int nz = T.nz;
   const int k = T.kw + kk;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
   const int kmin = T.kw < 0 ? 0 : T.kw;
   const unsigned kb = (unsigned)(kc - kmin) * (unsigned)sizeof(double) +
-                      (unsigned)(hh * S) * (unsigned)(dir == 0 ? nz * S_.ny : nz) * (unsigned)sizeof(double);
-  const int64_t stride = dir == 0 ? (int64_t)nz * S_.ny : (int64_t)nz;
-  const int64_t base = S_.off + (dir == 0 ? (int64_t)nz * T.line : (int64_t)nz * S_.ny * T.line) + kmin;
+                      (unsigned)(hh * S) * (unsigned)T.stride * (unsigned)sizeof(double);
+  const int64_t stride = T.stride;
+  const int64_t base = T.base;
 
-  // coefficient sets (identity rows beyond N), as the forward kernel
-  const int32_t* sids = v.setid + (dir == 0 ? S_.o_xset : S_.o_yset) + (int64_t)T.line * S_.nzone;
-  const int sid = sids[v.zone[S_.o_zone + kc]];
-  const int slot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : 0);
-  for (int q = 0; q < T.nset; ++q) {
-    const int sq = q == 0 ? T.set[0] : (q == 1 ? T.set[1] : T.set[2]);
-    const double* g = v.coef + v.set_off[sq];
-    double* dst = shC + q * AT::NA * NM;
-    for (int l = tid; l < NM; l += Cfg::THREADS) {
-      const bool ok = l < N;
-      if (AT::impl) {
-        dst[AT::A_BD * NM + l] = ok ? g[SET_BD * N + l] : 1.0;
-        dst[AT::A_FW * NM + l] = ok ? g[SET_FW * N + l] : 0.0;
-        dst[AT::A_BU * NM + l] = ok ? g[SET_BU * N + l] : 0.0;
-      }
-      if (AT::expl) {
-        dst[AT::A_EA * NM + l] = ok ? g[SET_EA * N + l] : 0.0;
-        dst[AT::A_EB * NM + l] = ok ? g[SET_EB * N + l] : 0.0;
+  // As the forward kernel: the coefficient sets (its padded copies: bd, fw,
+  // bu, ea, eb) and the rock flags go into LDS by LDS DMA, the per-element
+  // values through registers written after the line's loads are issued.
+  const int slot = (int)((T.slot_b0 >> kk) & 1u) | ((int)((T.slot_b1 >> kk) & 1u) << 1);
+  {
+    constexpr int NCH = AT::NA * NM / 2;  // 16-byte chunks of a slot
+    for (int q = 0; q < T.nset; ++q) {
+      const double* pg = v.pcoef[FAM] + (q == 0 ? T.poff[0] : (q == 1 ? T.poff[1] : T.poff[2]));
+      double* dst = shC + q * AT::NA * NM;
+#pragma unroll
+      for (int i = 0; i < (NCH + Cfg::THREADS - 1) / Cfg::THREADS; ++i) {
+        const int cw = i * Cfg::THREADS + wv * 64;  // the wave's first chunk
+        if (cw < NCH) {
+          const int ch = cw + lane;
+          if (ch < NCH) lds_dma16(pg + 2 * ch, dst + 2 * cw);
+        }
       }
     }
   }
-  // classes along the line, rock flags and z coefficients of the window's levels
+  // rock flags of the window's levels per column class: 4-byte DMA loads,
+  // lane i copying word i & 1 of level kw + i / 2 (+ 32 in the second load)
   const int ncls = S_.ncls < kZcMax ? S_.ncls : kZcMax;
-  for (int l = tid; l < NM; l += Cfg::THREADS) {
-    const int lc = l < N ? l : N - 1;
-    const int64_t col = dir == 0 ? (int64_t)lc * S_.ny + T.line : (int64_t)T.line * S_.ny + lc;
-    shK[l] = v.cls[S_.o_cls + col];
+  for (int q = wv; q < ncls; q += Cfg::NW)
+#pragma unroll
+    for (int half = 0; half < LW / 32; ++half) {
+      const int kq = T.kw + half * 32 + (lane >> 1);
+      const int kqc = kq < 0 ? 0 : (kq >= nz ? nz - 1 : kq);
+      lds_dma4(reinterpret_cast<const uint32_t*>(ag.rock + (int64_t)q * nz + kqc) + (lane & 1),
+               shR + q * LW + half * 32);
+    }
+  // classes, inverse face spacings and control lengths along the line
+  constexpr int NL = (NM + Cfg::THREADS - 1) / Cfg::THREADS;
+  int kv[NL];
+  double iv[NL], hv[NL];
+  {
     const double* gi = dir == 0 ? ag.ix : ag.iy;
     const double* gh = dir == 0 ? ag.hx : ag.hy;
-    shI[l] = l + 1 < N ? gi[l] : 0.0;
-    shH[l] = gh[lc];
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int l = tid + i * Cfg::THREADS;
+      const int lc = l < N ? l : N - 1;
+      const int64_t col = dir == 0 ? (int64_t)lc * S_.ny + T.line : (int64_t)T.line * S_.ny + lc;
+      kv[i] = l < NM ? v.cls[S_.o_cls + col] : 0;
+      iv[i] = l + 1 < N ? gi[l] : 0.0;
+      hv[i] = l < NM ? gh[lc] : 0.0;
+    }
   }
-  for (int q = wv; q < ncls; q += Cfg::NW)
-    if (hh == 0) shR[q * LW + kk] = ag.rock[(int64_t)q * nz + kc];
 
   double d[S];
   const int l0 = seg * S;
@@ -113,6 +126,15 @@ __global__ __launch_bounds__(Cfg::THREADS) void k_line_adj(DevView v, AdjGeo ag,
   {
 #pragma unroll
     for (int r = 0; r < S; ++r) d[r] = (l0 + r < N) ? *lane_ptr(in + soff(useg + (int64_t)r * ustride), kb) : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int l = tid + i * Cfg::THREADS;
+    if (l < NM) {
+      shK[l] = kv[i];
+      shI[l] = iv[i];
+      shH[l] = hv[i];
+    }
   }
   __syncthreads();
 
@@ -367,12 +389,52 @@ static hipError_t line_adj_cfg(const DevView& v, const AdjGeo& ag, int mode, dou
   return hipGetLastError();
 }
 
+// Tile shapes of the adjoint line kernels.  The forward kernels' tiles, with
+// the segments of the 33-320-element classes cut to 16-24 elements on twice
+// the waves: ADJ_X keeps its input line beside the solved one (one pass over
+// the forward field by linearity), which at 32 elements per lane held it to
+// two waves per SIMD.
+template <int LW> struct LineFamilyAdj;
+template <> struct LineFamilyAdj<64> {
+  using C0 = LineCfg<1, 48, 64>;
+  using C1 = LineCfg<4, 16, 64>;
+  using C2 = LineCfg<4, 24, 64>;
+  using C3 = LineCfg<8, 16, 64>;
+  using C4 = LineCfg<8, 20, 64>;
+  using C5 = LineCfg<8, 24, 64>;
+  using C6 = LineCfg<16, 16, 64>;
+  using C7 = LineCfg<16, 20, 64>;
+  using C8 = LineCfg<16, 26, 64>;
+  using C9 = LineCfg<8, 80, 64>;
+};
+template <> struct LineFamilyAdj<32> {
+  using C0 = LineCfg<1, 24, 32>;
+  using C1 = LineCfg<2, 16, 32>;
+  using C2 = LineCfg<2, 24, 32>;
+  using C3 = LineCfg<4, 16, 32>;
+  using C4 = LineCfg<4, 20, 32>;
+  using C5 = LineCfg<4, 24, 32>;
+  using C6 = LineCfg<8, 16, 32>;
+  using C7 = LineCfg<8, 20, 32>;
+  using C8 = LineCfg<8, 26, 32>;
+  using C9 = LineCfg<8, 40, 32>;
+};
+template <int LW>
+constexpr bool adj_family_matches_classes() {
+  using F = LineFamilyAdj<LW>;
+  return F::C0::NMAX == kLineClassMax[0] && F::C1::NMAX == kLineClassMax[1] && F::C2::NMAX == kLineClassMax[2] &&
+         F::C3::NMAX == kLineClassMax[3] && F::C4::NMAX == kLineClassMax[4] && F::C5::NMAX == kLineClassMax[5] &&
+         F::C6::NMAX == kLineClassMax[6] && F::C7::NMAX == kLineClassMax[7] && F::C8::NMAX == kLineClassMax[8] &&
+         F::C9::NMAX == kLineClassMax[9];
+}
+static_assert(adj_family_matches_classes<64>() && adj_family_matches_classes<32>(), "tile shapes follow the classes");
+
 // One launch per non-empty line-length class of the mode's tiles.
 template <int LW>
 hipError_t line_adj_family(const DevView& v, const AdjGeo& ag, int mode, double* out, const double* in,
                            const double* x1, const double* x2, double c, double a1, double a2, double a3,
                            hipStream_t st) {
-  using F = LineFamily<LW>;
+  using F = LineFamilyAdj<LW>;
   const int f = LW == 32 ? 1 : 0;
   const int64_t* cls = mode == ADJ_X ? v.xcls[f] : v.ycls[f];
   for (int q = 0; q < kLineClasses; ++q) {
