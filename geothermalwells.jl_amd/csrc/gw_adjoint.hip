@@ -255,10 +255,9 @@ __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPi
   double g = 0.0;
   for (int t = threadIdx.x; t < nl; t += blockDim.x) {
     const int lev = k0 + t;
-    const double hz = ag.hz[lev];
     const double x = ordered_sum_adj<kRedChunk>(R.ncol, [&](int c) {
       const AdjCol ac = scol[c];
-      return ac.hxy * hz * ag.rho[(int64_t)ac.cls * S.nz + lev] * mu[ac.off + lev];
+      return ac.hxy * ag.hzrho[(int64_t)ac.cls * S.nz + lev] * mu[ac.off + lev];
     });
     if (R.part == 0) ap.partial[R.poff + t] = x;
     else if (v.dep[B.dep_off + 2 * lev + 1].type == DEP_INLET) g += x;
@@ -275,7 +274,7 @@ __global__ __launch_bounds__(256) void k_adj_partial(DevView v, AdjGeo ag, AdjPi
 // + the annulus turnaround adjoint + the objective seed (inner outlet levels),
 // written back as mu = lambda_in / W.
 template <int R>
-__global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe ap, double* mu, const double* seed) {
+__global__ __launch_bounds__(256, R <= 3 ? 6 : 1) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe ap, double* mu, const double* seed) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* lout = reinterpret_cast<double*>(smem_raw);  // [countz_max]
   const RedChunk Gp = v.agroup[blockIdx.x];
@@ -284,20 +283,48 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
   const ScenDev S = v.sc[B.scen];
   const int nlev = S.countz;
   const int t = threadIdx.x;
+  // a column's mu and cell weights W = hxy hz rho at the thread's levels;
+  // the first column's go out before everything else (the gather lists and
+  // turnaround partials below overlap them), each next column's while the
+  // current one gathers
+  double pv[R], wv[R];
+  auto load_col = [&](const AdjCol& a) {
+    const double* hr = ag.hzrho + (int64_t)a.cls * S.nz;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = t + 256 * r;
+      pv[r] = k < nlev ? mu[a.off + k] : 0.0;
+      wv[r] = k < nlev ? hr[k] : 1.0;
+    }
+  };
+  AdjCol ac = ap.col[Gp.col0];
+  load_col(ac);
   const int* off = ap.off + (Gp.part == 0 ? A.csr_in : A.csr_an);
   const double sd = (Gp.part == 0 && seed) ? seed[Gp.bh] : 0.0;
   // the gather lists do not depend on the column: the first two entries of
-  // each of the thread's levels in registers, loaded once per block (a level
-  // of a semi-Lagrangian shift gathers from at most a few departures)
+  // each level, staged in LDS once per block (a level of a semi-Lagrangian
+  // shift gathers from at most a few departures); the rest of a longer list
+  // is read from the table
+  int* gk = reinterpret_cast<int*>(lout + 2 * (size_t)v.countz_max);   // [2][countz_max]
+  double* gw = lout + 3 * (size_t)v.countz_max;                         // [2][countz_max]
   int e_beg[R], e_end[R];
-  AdjEnt g0[R], g1[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int k = t + 256 * r;
     e_beg[r] = k < nlev ? off[k] : 0;
     e_end[r] = k < nlev ? off[k + 1] : 0;
-    g0[r] = e_end[r] > e_beg[r] ? ap.ent[e_beg[r]] : AdjEnt{0, 0, 0.0};
-    g1[r] = e_end[r] > e_beg[r] + 1 ? ap.ent[e_beg[r] + 1] : AdjEnt{0, 0, 0.0};
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k = t + 256 * r;
+    if (k < nlev) {
+      const AdjEnt a = e_end[r] > e_beg[r] ? ap.ent[e_beg[r]] : AdjEnt{0, 0, 0.0};
+      const AdjEnt b = e_end[r] > e_beg[r] + 1 ? ap.ent[e_beg[r] + 1] : AdjEnt{0, 0, 0.0};
+      gk[k] = a.k;
+      gk[v.countz_max + k] = b.k;
+      gw[k] = a.w;
+      gw[v.countz_max + k] = b.w;
+    }
   }
   // the group's column entries through LDS (a vector load of the table in
   // the loop would wait for every access in flight)
@@ -332,22 +359,22 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
     }
   }
   for (int c = 0; c < Gp.ncol; ++c) {
-    const AdjCol ac = scol[c];
     double* p = mu + ac.off;
-    const double* rho = ag.rho + (int64_t)ac.cls * S.nz;
     double wk[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = t + 256 * r;
-      wk[r] = k < nlev ? ac.hxy * ag.hz[k] * rho[k] : 1.0;
-      if (k < nlev) lout[k] = wk[r] * p[k];
+      wk[r] = k < nlev ? ac.hxy * wv[r] : 1.0;
+      if (k < nlev) lout[k] = wk[r] * pv[r];
     }
+    const AdjCol an = c + 1 < Gp.ncol ? scol[c + 1] : ac;
+    if (c + 1 < Gp.ncol) load_col(an);  // distinct columns: no overlap with this column's stores
     lds_barrier();  // distinct columns: only lout needs the barrier
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int k = t + 256 * r;
       if (k < nlev) {
-        double lin = g0[r].w * lout[g0[r].k] + g1[r].w * lout[g1[r].k];
+        double lin = gw[k] * lout[gk[k]] + gw[v.countz_max + k] * lout[gk[v.countz_max + k]];
         for (int e = e_beg[r] + 2; e < e_end[r]; ++e) lin += ap.ent[e].w * lout[ap.ent[e].k];
         lin += sk[r];
         if (Gp.part == 0 && k < B.n_outlev) lin += sd;
@@ -356,9 +383,12 @@ __global__ __launch_bounds__(256) void k_adj_apply(DevView v, AdjGeo ag, AdjPipe
     }
     // outlet levels below the borehole bottom (v_inner dt/2 > h): cells the
     // advection leaves alone but the outlet mean still reads
-    if (sd != 0.0)
-      for (int k = nlev + t; k < B.n_outlev; k += 256) p[k] += sd / (ac.hxy * ag.hz[k] * rho[k]);
+    if (sd != 0.0) {
+      const double* hr = ag.hzrho + (int64_t)ac.cls * S.nz;
+      for (int k = nlev + t; k < B.n_outlev; k += 256) p[k] += sd / (ac.hxy * hr[k]);
+    }
     lds_barrier();
+    ac = an;
   }
 }
 
@@ -399,7 +429,10 @@ hipError_t launch_adj_pipe(const DevView& v, const AdjGeo& ag, const AdjPipe& ap
     if (e != hipSuccess) return e;
   }
   if (v.nagroup == 0) return hipSuccess;
-  const size_t sh = 2 * (size_t)v.countz_max * sizeof(double);  // lout, lam
+  // lout, lam, the gather lists' first two source levels (int) and weights:
+  // 40 bytes per level (24 KB at 602 levels, six blocks per CU)
+  const size_t sh = 2 * (size_t)v.countz_max * sizeof(double) + 2 * (size_t)v.countz_max * sizeof(int) +
+                    2 * (size_t)v.countz_max * sizeof(double);
   const int R = (v.countz_max + 255) / 256;
   const dim3 grid((unsigned)v.nagroup), block(256);
   if (R <= 1) hipLaunchKernelGGL((k_adj_apply<1>), grid, block, sh, st, v, ag, ap, mu, seed);

@@ -1774,6 +1774,9 @@ AdjGeo make_adj_geo(const gw_ctx& ctx, DevBufs& db) {
   ag.hx = db.up(hx); ag.hy = db.up(hy); ag.hz = db.up(hz);
   ag.ix = db.up(ix); ag.iy = db.up(iy); ag.iz = db.up(iz);
   ag.rho = db.up(s.rprof);
+  std::vector<double> hzrho(s.rprof.size());
+  for (size_t q = 0; q < hzrho.size(); ++q) hzrho[q] = hz[q % (size_t)s.nz] * s.rprof[q];
+  ag.hzrho = db.up(hzrho);
   ag.rock = db.up(rock);
   ag.yparts = adj_yparts(ctx.hcls[0][kLineClasses], s.ny);
   ag.ncls = s.ncls;
@@ -2071,14 +2074,19 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     // storage: checkpoints, one segment of u / e, two adjoint fields, two work fields
     const int nseg = (nsteps + seg - 1) / seg;
     const int sl = std::min(seg, nsteps);
+    // all of them in one allocation (a few hundred separate hipMalloc/hipFree
+    // pairs of field-sized buffers cost ~2% of a 30-day gradient)
     std::vector<double*> ck(nseg), U(sl + 1), E(sl);
-    for (auto& p : ck) p = db.alloc<double>(cells);
-    for (auto& p : U) p = db.alloc<double>(cells);
-    for (auto& p : E) p = db.alloc<double>(cells);
-    double* MU0 = db.alloc<double>(cells);
-    double* MU1 = db.alloc<double>(cells);
-    double* WB = db.alloc<double>(cells);
-    double* WB2 = db.alloc<double>(cells);
+    const size_t fstride = (cells + 31) / 32 * 32;  // 256-byte aligned fields
+    double* arena = db.alloc<double>(fstride * (size_t)(nseg + (sl + 1) + sl + 4));
+    size_t nf = 0;
+    for (auto& p : ck) p = arena + fstride * nf++;
+    for (auto& p : U) p = arena + fstride * nf++;
+    for (auto& p : E) p = arena + fstride * nf++;
+    double* MU0 = arena + fstride * nf++;
+    double* MU1 = arena + fstride * nf++;
+    double* WB = arena + fstride * nf++;
+    double* WB2 = arena + fstride * nf++;
     double* jacc = db.up(std::vector<double>(nbh, 0.0));
     // every sub-step's level means and inlets of the forward run, for the
     // recompute (up to 1 GiB; beyond that the recompute reduces again)

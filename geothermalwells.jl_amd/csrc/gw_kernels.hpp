@@ -131,6 +131,7 @@ struct AdjGeo {
   const double* iy;      // [ny]
   const double* iz;      // [nz]
   const double* rho;     // [ncls][nz] volumetric heat capacity per column class
+  const double* hzrho;   // [ncls][nz] hz[k] rho[cls][k] (a pipe cell's weight W over its hx hy)
   const double* rock;    // [ncls][nz] 1 where the conductivity is the rock layer's, else 0
   double* gx_part;       // [ny][nz] conductivity-gradient sums of the x-line kernels (per cell level)
   double* gy_part;       // [yparts][nx][nz] ... of the y-line kernels (y faces and z faces); the
