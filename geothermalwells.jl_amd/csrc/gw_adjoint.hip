@@ -73,12 +73,8 @@ __global__ __launch_bounds__(64 * kStreamWaves, 8) void k_adj_yz(DevView v, AdjG
   const int k = T.kw + lane;
   const int kc = k < 0 ? 0 : (k >= nz ? nz - 1 : k);
   const int ncls = ncls_l;
-  // the lane's explicit-y set (zone -> set id: its slot among the tile's sets)
-  int eslot = 0;
-  if (EY) {
-    const int sid = v.setid[S.o_yset + (int64_t)i * S.nzone + v.zone[S.o_zone + kc]];
-    eslot = sid == T.set[1] ? 1 : (sid == T.set[2] ? 2 : (sid == T.set[3] ? 3 : 0));
-  }
+  // the lane's explicit-y set: its slot among the tile's sets (from the tile)
+  const int eslot = EY ? (int)((T.slot_b0 >> lane) & 1u) | ((int)((T.slot_b1 >> lane) & 1u) << 1) : 0;
   // block's elements j (clamped) from jlo: la - 1 .. lb + 2 of its waves
   const int jlo = max(0, p * kStreamWaves * kWalk - 1);
   for (int q = threadIdx.x; q < kJs; q += 64 * kStreamWaves) {
@@ -87,14 +83,17 @@ __global__ __launch_bounds__(64 * kStreamWaves, 8) void k_adj_yz(DevView v, AdjG
     shIy[q] = ag.iy[jj > 0 ? jj - 1 : 0];
     shCl[q] = v.cls[S.o_cls + (int64_t)i * S.ny + jj];
   }
-  if (EY)
+  if (EY) {
+    // from the sets' padded copies (the line kernels' layout: ea, eb are
+    // arrays 3 and 4 of the set's NM-element arrays), addressed from the tile
+    const int NM = line_class_max(N);
     for (int q = threadIdx.x; q < T.nset * kJs; q += 64 * kStreamWaves) {
       const int s = q / kJs, jj = min(jlo + q - s * kJs, N - 1);
-      const int sq = s == 0 ? T.set[0] : (s == 1 ? T.set[1] : (s == 2 ? T.set[2] : T.set[3]));
-      const double* g = v.coef + v.set_off[sq];
-      shEa[q] = g[SET_EA * N + jj];
-      shEb[q] = g[SET_EB * N + jj];
+      const double* g = v.pcoef[0] + (s == 0 ? T.poff[0] : (s == 1 ? T.poff[1] : (s == 2 ? T.poff[2] : T.poff[3])));
+      shEa[q] = g[3 * NM + jj];
+      shEb[q] = g[4 * NM + jj];
     }
+  }
   // z coefficients and rock terms of the column classes: all of a wave's loads
   // before its LDS writes (one round trip)
   {

@@ -33,6 +33,16 @@ GW_DEV void lds_barrier() {
   asm volatile("" ::: "memory");
 }
 
+// kLineClassMax[line_class(n)] on the device: the longest line of n's
+// line-length class (the padded length of its coefficient sets).
+GW_DEV int line_class_max(int n) {
+  int m = kLineClassMax[kLineClasses - 1];
+#pragma unroll
+  for (int q = kLineClasses - 1; q >= 0; --q)
+    if (n <= kLineClassMax[q]) m = kLineClassMax[q];
+  return m;
+}
+
 // LDS DMA (global_load_lds_dwordx4): each active lane copies the 16 bytes at
 // g straight into LDS, lane i to lds_wave + 16 i (lds_wave wave-uniform), no
 // registers involved.  It completes under vmcnt; a __syncthreads() after it
