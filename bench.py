@@ -528,16 +528,19 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = cells * args.steps / wall
     T_out = cache.T_outlet
-    assert np.all(np.isfinite(T_out)), "non-finite outlet temperatures"
-    # finiteness of the state through its sum (NaN and Inf propagate): an
-    # elementwise isfinite() allocates ~1.5 GB of temporaries, after which the
-    # line kernels run 10-20% slower for several launches on these boxes
-    state_ok = bool(np.isfinite(float(u.sum().item())))
 
-    # per-operator HIP-event timing on the context's stream
+    # per-operator HIP-event timing on the context's stream, after a few
+    # steps queued right before it: the GPU lowers its clocks within
+    # milliseconds of idling (the host's checks above), and the first
+    # launches after an idle gap would run the fused y kernel 10-20% slower
     ms = (ctypes.c_double * 6)()
     t0 = (args.warmup + args.steps) * dt
-    gw._abi.check(lib.gw_profile_steps(cache.ctx, ptr, t0, dt, args.profile_steps, ms))
+    gw._abi.check(lib.gw_steps(cache.ctx, ptr, t0, dt, 6))
+    gw._abi.check(lib.gw_profile_steps(cache.ctx, ptr, t0 + 6 * dt, dt, args.profile_steps, ms))
+    assert np.all(np.isfinite(T_out)), "non-finite outlet temperatures"
+    # finiteness of the state through its sum (NaN and Inf propagate; an
+    # elementwise isfinite() would allocate ~1.5 GB of temporaries)
+    state_ok = bool(np.isfinite(float(u.sum().item())))
     n_p = args.profile_steps
     launches = [1, n_p, n_p, n_p, n_p - 1, 1]
     per_op = {OPS[q]: ms[q] / launches[q] for q in range(6) if launches[q] > 0}
