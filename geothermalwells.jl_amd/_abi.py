@@ -14,6 +14,7 @@ GW_F64, GW_F32_STATE, GW_F32_STATE_SR = 0, 1, 2
 GW_FLAG_LINE_W64, GW_FLAG_LINE_W32, GW_FLAG_LINE_TWO_TILES, GW_FLAG_HALO_WHOLE = 1, 2, 4, 8
 GW_FLAG_LINE_CONCURRENT = 16
 GW_FLAG_STEPS_GRAPH, GW_FLAG_STEPS_DIRECT = 32, 64
+GW_FLAG_KEEP_LINE_CLASSES = 128
 GW_INLET_CONSTANT, GW_INLET_HEAT_EXCHANGER, GW_INLET_PER_WELL, GW_INLET_HOST = 0, 1, 2, 3
 
 c_double_p = ctypes.POINTER(ctypes.c_double)

@@ -98,7 +98,7 @@ class Cache:
 
     def __init__(self, scenarios, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
                  line_window=None, line_two_tiles=False, line_halo_whole=False, line_concurrent=False,
-                 steps_launch=None):
+                 steps_launch=None, keep_line_classes=False):
         self._scen = scenarios
         self.backend = torch.device(backend) if backend is not None else torch.device("cuda")
         if self.backend.type != "cuda":
@@ -128,6 +128,8 @@ class Cache:
         if steps_launch not in (None, "graph", "direct"):
             raise ValueError("steps_launch is None (timed on the first long call), 'graph' or 'direct'")
         prob.flags |= {None: 0, "graph": _abi.GW_FLAG_STEPS_GRAPH, "direct": _abi.GW_FLAG_STEPS_DIRECT}[steps_launch]
+        if keep_line_classes:
+            prob.flags |= _abi.GW_FLAG_KEEP_LINE_CLASSES
         h = ctypes.c_void_p()
         _abi.check(lib.gw_create(ctypes.byref(prob), ctypes.byref(h)))
         self._ctx = h
@@ -244,14 +246,18 @@ def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inl
 
 
 def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                       line_window=None, line_concurrent=False, steps_launch=None):
+                       line_window=None, line_concurrent=False, steps_launch=None, keep_line_classes=False):
     """Many independent scenarios (each a dict with the create_cache keywords
     gridx, gridy, gridz, materials, boreholes, inlet_model) advanced together:
-    every kernel launch covers all of them."""
+    every kernel launch covers all of them.  Line-length classes holding under
+    5% of the batch's line cells run with the next larger class's tiles unless
+    `keep_line_classes` (the results differ by rounding only: other segment
+    lengths split a line's recurrences elsewhere)."""
     sc = [_Scenario(d["gridx"], d["gridy"], d["gridz"], d["materials"], d["boreholes"], d["inlet_model"])
           for d in scenarios]
     return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3,
-                 line_window=line_window, line_concurrent=line_concurrent, steps_launch=steps_launch)
+                 line_window=line_window, line_concurrent=line_concurrent, steps_launch=steps_launch,
+                 keep_line_classes=keep_line_classes)
 
 
 # ---------------------------------------------------------------------------

@@ -266,6 +266,10 @@ struct gw_ctx {
   // the width the problem flags force
   int lw_sel[5][2] = {};
   int force_lw = 0;
+  // line-length class whose tile shape runs each class's lines (identity
+  // unless small classes of a ragged batch are merged into larger ones)
+  int cls_map[kLineClasses] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+  int line_cls(int n) const { return cls_map[line_class(n)]; }
   bool tuned = false;
   int steps_mode = kStepsUndecided;     // gw_steps: graph replays or direct launches (kSteps*)
   int adj_lw[2] = {0, 0};  // k-window width of the adjoint line modes ADJ_Y, ADJ_X (0: not yet timed)
@@ -500,14 +504,41 @@ void build_line_sets(gw_ctx& ctx, Scen& s, std::unordered_map<std::string, int>&
 // Offsets (in doubles) of each set's padded copy in DevView::pcoef: SET_NARR
 // arrays of its line-length class's longest line.  They depend on the sets'
 // lengths only, so every table of the context and the tiles agree on them.
-std::vector<int64_t> padded_set_offsets(const std::vector<LineSet>& sets) {
-  std::vector<int64_t> off(sets.size());
+std::vector<int64_t> padded_set_offsets(const gw_ctx& ctx) {
+  std::vector<int64_t> off(ctx.sets.size());
   int64_t tot = 0;
-  for (size_t q = 0; q < sets.size(); ++q) {
+  for (size_t q = 0; q < ctx.sets.size(); ++q) {
     off[q] = tot;
-    tot += SET_NARR * (int64_t)kLineClassMax[line_class(sets[q].N)];
+    tot += SET_NARR * (int64_t)kLineClassMax[ctx.line_cls(ctx.sets[q].N)];
   }
   return off;
+}
+
+// Merges the line-length classes of a ragged batch that hold under
+// kClassMergeShare of its line cells, in x and in y, into the next larger
+// class present: their lines then run with that class's tile shape (longer
+// segments or more waves, the padding rows idle) in its launch, instead of a
+// launch of their own whose few tiles run latency-bound.
+constexpr double kClassMergeShare = 0.05;
+void merge_small_classes(gw_ctx& ctx) {
+  double cells[2][kLineClasses] = {}, tot[2] = {};
+  for (const Scen& s : ctx.sc) {
+    const double n = (double)s.nx * s.ny * s.nz;
+    cells[0][line_class(s.nx)] += n;
+    cells[1][line_class(s.ny)] += n;
+    tot[0] += n;
+    tot[1] += n;
+  }
+  for (int q = 0; q < kLineClasses; ++q) {
+    if (cells[0][q] == 0 && cells[1][q] == 0) continue;
+    if (cells[0][q] >= kClassMergeShare * tot[0] || cells[1][q] >= kClassMergeShare * tot[1]) continue;
+    int q2 = q + 1;
+    while (q2 < kLineClasses && cells[0][q2] == 0 && cells[1][q2] == 0) ++q2;
+    if (q2 == kLineClasses) break;
+    for (int d = 0; d < 2; ++d) { cells[d][q2] += cells[d][q]; cells[d][q] = 0; }
+    for (int r = 0; r < kLineClasses; ++r)
+      if (ctx.cls_map[r] == q) ctx.cls_map[r] = q2;
+  }
 }
 
 void line_tiles(const Scen& s, int si, int dir, int halo, int lw, const std::vector<int64_t>& pset_off,
@@ -644,15 +675,15 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
   // states: ea, eb.  FP32 states: the explicit step's products c ea, c eb,
   // as doubles for the 32-lane family and rounded to float (an array of NM
   // floats at the start of each NM-double array) for the 64-lane family.
-  const std::vector<int64_t> pset_off = padded_set_offsets(ctx.sets);
-  const int64_t ptot = ctx.sets.empty() ? 0 : pset_off.back() + SET_NARR * (int64_t)kLineClassMax[line_class(ctx.sets.back().N)];
+  const std::vector<int64_t> pset_off = padded_set_offsets(ctx);
+  const int64_t ptot = ctx.sets.empty() ? 0 : pset_off.back() + SET_NARR * (int64_t)kLineClassMax[ctx.line_cls(ctx.sets.back().N)];
   const bool f32 = ctx.prec != GW_F64;
   std::vector<double> pcoef[2];
   for (int f = 0; f < (f32 ? 2 : 1); ++f) {
     pcoef[f].assign((size_t)std::max<int64_t>(ptot, 2), 0.0);
     const double cs = f32 ? c : 1.0;
     for (size_t q = 0; q < ctx.sets.size(); ++q) {
-      const int N = ctx.sets[q].N, NM = kLineClassMax[line_class(N)];
+      const int N = ctx.sets[q].N, NM = kLineClassMax[ctx.line_cls(N)];
       const double* g = &coef[set_off[q]];
       double* p = &pcoef[f][pset_off[q]];
       for (int l = 0; l < NM; ++l) {
@@ -974,7 +1005,8 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
   const int32_t width = p->flags & (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32);
   const int32_t variant = p->flags & (GW_FLAG_LINE_TWO_TILES | GW_FLAG_HALO_WHOLE | GW_FLAG_LINE_CONCURRENT);
   const int32_t steps = p->flags & (GW_FLAG_STEPS_GRAPH | GW_FLAG_STEPS_DIRECT);
-  if ((p->flags & ~(width | variant | steps)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) ||
+  const int32_t keep_classes = p->flags & GW_FLAG_KEEP_LINE_CLASSES;
+  if ((p->flags & ~(width | variant | steps | keep_classes)) || width == (GW_FLAG_LINE_W64 | GW_FLAG_LINE_W32) ||
       (variant && !width) || steps == (GW_FLAG_STEPS_GRAPH | GW_FLAG_STEPS_DIRECT))
     return fail(GW_EINVAL, "unknown or conflicting flags (the launch variants need a forced width)");
   std::unique_ptr<gw_ctx> ctx(new gw_ctx());
@@ -1084,7 +1116,8 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       }
     std::vector<ScenDev> sd;
     std::vector<int64_t> cellp{0}, colp{0};
-    const std::vector<int64_t> pset_off = padded_set_offsets(ctx->sets);
+    if (!keep_classes) merge_small_classes(*ctx);
+    const std::vector<int64_t> pset_off = padded_set_offsets(*ctx);
     if (!pset_off.empty() && pset_off.back() > (int64_t)UINT32_MAX / 2)
       throw std::runtime_error("too many coefficient sets");
     std::vector<LineTile> xb[2][kLineClasses], yb[2][kLineClasses], hb[2][kLineClasses];
@@ -1106,8 +1139,9 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
       const int64_t ncell = (int64_t)s.nx * s.ny * s.nz;
       cellp.push_back(cellp.back() + ncell);
       colp.push_back(colp.back() + (int64_t)s.nx * s.ny);
-      const int qx = line_class(s.nx), qy = line_class(s.ny);
-      if (qx < 0 || qy < 0) throw std::runtime_error("grid lines longer than the line kernels support");
+      if (line_class(s.nx) < 0 || line_class(s.ny) < 0)
+        throw std::runtime_error("grid lines longer than the line kernels support");
+      const int qx = ctx->line_cls(s.nx), qy = ctx->line_cls(s.ny);
       for (int f = 0; f < 2; ++f) {
         const int lw = f == 0 ? 64 : 32;
         line_tiles(s, (int)sd.size() - 1, 0, 0, lw, pset_off, xb[f][qx]);

@@ -57,10 +57,16 @@ enum { GW_F64 = 0, GW_F32_STATE = 1, GW_F32_STATE_SR = 2 };
  * line-length classes of a batch concurrently on side streams.  gw_steps runs
  * the steps of a call by replaying one captured hipGraph per step or by
  * launching the same kernels directly; the first call of 8 or more steps times
- * both and keeps the faster, GW_FLAG_STEPS_GRAPH / _DIRECT force one.  Results
- * do not depend on these choices. */
+ * both and keeps the faster, GW_FLAG_STEPS_GRAPH / _DIRECT force one.  In a
+ * ragged batch, a line-length class holding under 5% of the batch's line
+ * cells (in x and in y) runs with the next larger class's tile shape, one
+ * launch less per operator; GW_FLAG_KEEP_LINE_CLASSES gives every class its
+ * own launch.  Results do not depend on these choices beyond rounding: a tile
+ * shape with other segment lengths splits the segmented recurrences of a line
+ * elsewhere. */
 enum { GW_FLAG_LINE_W64 = 1, GW_FLAG_LINE_W32 = 2, GW_FLAG_LINE_TWO_TILES = 4, GW_FLAG_HALO_WHOLE = 8,
-       GW_FLAG_LINE_CONCURRENT = 16, GW_FLAG_STEPS_GRAPH = 32, GW_FLAG_STEPS_DIRECT = 64 };
+       GW_FLAG_LINE_CONCURRENT = 16, GW_FLAG_STEPS_GRAPH = 32, GW_FLAG_STEPS_DIRECT = 64,
+       GW_FLAG_KEEP_LINE_CLASSES = 128 };
 
 /* Inlet models (src/physics.jl:283-318). */
 enum {

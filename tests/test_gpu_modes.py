@@ -51,6 +51,59 @@ def test_batched_ragged_matches_oracle_per_scenario():
         assert abs(T[s] - hist[-1][0]) <= 1e-6, s
 
 
+def _class_shares(ws):
+    """Share of the batch's cells in the lines of each line-length class (x
+    and y), the measure by which the library merges small classes."""
+    cmax = (48, 64, 96, 128, 160, 192, 256, 320, 416, 640)
+    sh = [[0.0] * len(cmax), [0.0] * len(cmax)]
+    tot = 0.0
+    for w in ws:
+        nx, ny, nz = len(w["gridx"]), len(w["gridy"]), len(w["gridz"])
+        sh[0][next(q for q, m in enumerate(cmax) if nx <= m)] += nx * ny * nz
+        sh[1][next(q for q, m in enumerate(cmax) if ny <= m)] += nx * ny * nz
+        tot += nx * ny * nz
+    return [[x / tot for x in d] for d in sh]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batched_small_class_merged(prec):
+    """A batch whose 34x32-point scenario holds under 5% of the cells: its
+    lines run with the next larger class's tiles (longer segments, padding
+    rows).  Per scenario it matches the oracle as a batch with every class
+    launched on its own does, and the two agree to rounding."""
+    torch = _torch()
+    g = H.gw()
+    ws = [H.small_case(h=75.0, n_fine=0.004, nz_dz=(80.0, 2.0)),
+          H.small_case(h=40.0, n_fine=0.015, nz_dz=(46.0, 2.0), inlet="hx"),
+          H.small_case(h=60.0, n_fine=0.006, layered=True),
+          H.small_case(h=60.0, n_fine=0.004, nz_dz=(70.0, 2.0), inlet="perwell")]
+    sh = _class_shares(ws)
+    assert 0 < sh[0][0] < 0.05 and 0 < sh[1][0] < 0.05  # the 34x32 scenario's class is merged
+    out = {}
+    for keep in (False, True):
+        cache = g.create_batch_cache([_kw(w) for w in ws], precision=prec, keep_line_classes=keep)
+        u = cache.new_state()
+        for s, w in enumerate(ws):
+            u0 = H.oracle_for(w).initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+            cache.scenario_view(u, s).copy_(torch.from_numpy(u0))
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, 10))
+        out[keep] = (u.double().cpu().numpy(), cache.T_outlet, cache)
+    um, Tm, cm = out[False]
+    uk, Tk, _ = out[True]
+    if prec == "f64":
+        assert H.rel_err(um, uk) <= 1e-12
+        assert np.abs(Tm - Tk).max() <= 1e-9
+        for s, w in enumerate(ws):
+            orc = H.oracle_for(w)
+            u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+            ref, hist, _ = orc.run(u0, 0.0, 60.0, 10, cm.rock2_c3)
+            assert H.rel_err(cm.scenario_view(torch.from_numpy(um), s).numpy(), ref) <= 1e-10, s
+            assert abs(Tm[s] - hist[-1][0]) <= 1e-6, s
+    else:
+        assert H.rel_err(um, uk) <= 1e-5
+        assert np.abs(Tm - Tk).max() <= 1e-4
+
+
 def test_fp32_state_outlet_error():
     """FP32 field / stencil / advection with FP64 elimination and reductions:
     the outlet stays within 1e-3 K of the FP64 oracle."""
