@@ -207,6 +207,8 @@ struct CTab {
   Dep* d_dep = nullptr;
   double* d_means = nullptr;
   RedChunk* d_rchunk = nullptr;
+  PartChunk* d_pchunk = nullptr;
+  int32_t* d_pblock = nullptr;
   RedChunk* d_agroup = nullptr;
   double* d_partial = nullptr;
   std::vector<Dep> h_dep;           // host copy of the departure table (adjoint)
@@ -214,6 +216,7 @@ struct CTab {
   int nmeans = 0;
   int npartial = 0;                 // doubles of d_partial
   int nrchunk = 0;
+  int npblock = 0;
   int nagroup = 0;
 };
 
@@ -253,6 +256,7 @@ struct gw_ctx {
   float* d_zcoef32 = nullptr;       // the same rounded to float (FP32 states)
   int32_t* d_setid = nullptr;
   PipeCol* d_pcol = nullptr;
+  int64_t* d_pcol_off = nullptr;    // per pipe column: (j + ny i) nz, its level-0 offset in its scenario
   double *d_tout = nullptr, *d_inlet = nullptr, *d_inlet_tab = nullptr;
   uint32_t* d_step_ctr = nullptr;  // GW_F32_STATE_SR: step index keying the stochastic rounding
   int nx_max = 0, ny_max = 0, nz_max = 0, countz_max = 0;
@@ -315,6 +319,7 @@ struct gw_ctx {
       (void)hipFree(kv.second.d_pcoef[0]); (void)hipFree(kv.second.d_pcoef[1]);
       (void)hipFree(kv.second.d_dep); (void)hipFree(kv.second.d_means);
       (void)hipFree(kv.second.d_rchunk); (void)hipFree(kv.second.d_partial);
+      (void)hipFree(kv.second.d_pchunk); (void)hipFree(kv.second.d_pblock);
       (void)hipFree(kv.second.d_agroup);
     }
     for (void* p : allocs) (void)hipFree(p);
@@ -346,8 +351,9 @@ struct gw_ctx {
     if (t) { v.set_off = t->d_set_off; v.coef = t->d_coef; v.pcoef[0] = t->d_pcoef[0]; v.pcoef[1] = t->d_pcoef[1] ? t->d_pcoef[1] : t->d_pcoef[0];
              v.bh = t->d_bh; v.dep = t->d_dep; v.means = t->d_means;
              v.rchunk = t->d_rchunk; v.nrchunk = t->nrchunk; v.partial = t->d_partial;
+             v.pchunk = t->d_pchunk; v.pblock = t->d_pblock; v.npblock = t->npblock;
              v.agroup = t->d_agroup; v.nagroup = t->nagroup; }
-    v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size();
+    v.nbh = nbh_total; v.pcol = d_pcol; v.npcol = (int)pcol.size(); v.pcol_off = d_pcol_off;
     v.tout = d_tout; v.inlet = d_inlet; v.inlet_tab = d_inlet_tab;
     v.step_ctr = d_step_ctr;
     v.nx_max = nx_max; v.ny_max = ny_max; v.nz_max = nz_max; v.countz_max = countz_max;
@@ -817,6 +823,27 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
         groups.push_back(RedChunk{gb, 1, B.col_an0 + c0, std::min(kAdvGroup, B.n_an - c0), 0, 0});
     }
   }
+  // the partial-sum kernel's view of the chunks, packed into blocks of
+  // consecutive chunks with at most kPartItems levels in all
+  std::vector<PartChunk> pch(chunks.size());
+  std::vector<int32_t> pblock;
+  int items = 0;
+  for (size_t q = 0; q < chunks.size(); ++q) {
+    const RedChunk& R = chunks[q];
+    const BhDev& B = hb[R.bh];
+    const int nl = R.part == 0 ? B.n_outlev : B.nlev;
+    const int nq = pblock.empty() ? 0 : (int)q - pblock.back();
+    if (pblock.empty() || nq == kPartChunks || items + nl > kPartItems || items > kPartItems) {
+      pblock.push_back((int32_t)q);
+      items = 0;
+    }
+    PartChunk& P = pch[q];
+    P.base = ctx.sc[B.scen].off + (R.part == 0 ? 0 : B.klo);
+    P.col0 = R.col0; P.ncol = R.ncol; P.nl = nl; P.poff = R.poff; P.item0 = items; P.pad = 0;
+    items += nl;
+  }
+  const int npblock = (int)pblock.size();
+  pblock.push_back((int32_t)chunks.size());
   try {
     t.d_set_off = dev_upload(set_off);
     t.d_coef = dev_upload(coef);
@@ -829,6 +856,9 @@ int build_ctab(gw_ctx& ctx, double c, CTab*& out) {
     t.nmeans = mean_off;
     t.d_rchunk = dev_upload(chunks);
     t.nrchunk = (int)chunks.size();
+    t.d_pchunk = dev_upload(pch);
+    t.d_pblock = dev_upload(pblock);
+    t.npblock = npblock;
     t.d_agroup = dev_upload(groups);
     t.nagroup = (int)groups.size();
     std::vector<double> part(std::max(poff, 1), 0.0);
@@ -1198,6 +1228,13 @@ int create_impl(const gw_problem* p, gw_ctx** out) {
     }
     ctx->d_setid = ctx->up(setid);
     ctx->d_pcol = ctx->up(ctx->pcol);
+    std::vector<int64_t> pcol_off(ctx->pcol.size());
+    for (size_t q = 0; q < ctx->pcol.size(); ++q) {
+      const PipeCol& pc = ctx->pcol[q];
+      const Scen& s = ctx->sc[ctx->bh_scen[pc.bh]];
+      pcol_off[q] = ((int64_t)pc.j + (int64_t)s.ny * pc.i) * s.nz;
+    }
+    ctx->d_pcol_off = ctx->up(pcol_off);
     std::vector<double> zeros(std::max(ctx->nbh_total, 1), 0.0);
     ctx->d_tout = ctx->up(zeros);
     ctx->d_inlet = ctx->up(zeros);

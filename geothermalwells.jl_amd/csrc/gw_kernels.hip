@@ -165,29 +165,44 @@ GW_DEV void pipe_final(const DevView& v, int b, double* red) {
   }
 }
 
-// Pass 1: one block per reduction chunk (up to kRedChunk pipe columns of one
-// borehole); thread t owns level t (coalesced within each column) and sums the
-// chunk's columns in order, all loads in flight at once.  (A single pass with
-// a last-block combine would need device-scope fences, which on this multi-XCD
-// part write back and invalidate the whole L2 per block.)
+// Pass 1: each block sums the levels of its chunks (up to kPartChunks
+// consecutive chunks of at most kPartItems levels in all, one thread per
+// chunk level; a longer chunk alone, thread t owning levels t, t + 256, ...):
+// a level's sum adds the chunk's columns in order, all loads in flight at
+// once (coalesced within each column).  Small chunks share a block so a
+// batch of many shallow reductions is not a launch of mostly idle blocks.
+// (A single pass with a last-block combine would need device-scope fences,
+// which on this multi-XCD part write back and invalidate the whole L2 per
+// block.)
 template <class TS>
 __global__ __launch_bounds__(256) void k_pipe_partial(DevView v, const TS* __restrict__ phi) {
-  const RedChunk R = v.rchunk[blockIdx.x];
-  const BhDev B = v.bh[R.bh];
-  const ScenDev S = v.sc[B.scen];
-  const int k0 = R.part == 0 ? 0 : B.klo;
-  const int nl = R.part == 0 ? B.n_outlev : B.nlev;
-  const TS* base = phi + S.off + k0;
-  // the chunk's column offsets once, through LDS (in the level loop each
-  // column would cost a table load before its value load)
-  __shared__ int64_t coff[kRedChunk];
-  if ((int)threadIdx.x < R.ncol) {
-    const PipeCol pc = v.pcol[R.col0 + threadIdx.x];
-    coff[threadIdx.x] = ((int64_t)pc.j + (int64_t)S.ny * pc.i) * S.nz;
+  const int c0 = v.pblock[blockIdx.x], nc = v.pblock[blockIdx.x + 1] - c0;
+  const int t = threadIdx.x;
+  // the chunks' column offsets and headers through LDS (in the level loop
+  // each column would cost a table load before its value load)
+  __shared__ int64_t coff[kPartChunks * kRedChunk];
+  __shared__ int32_t item0[kPartChunks], nl[kPartChunks], ncol[kPartChunks], poff[kPartChunks];
+  if (t < nc * kRedChunk) {
+    const PartChunk P = v.pchunk[c0 + t / kRedChunk];
+    const int c = t % kRedChunk;
+    if (c < P.ncol) coff[t] = P.base + v.pcol_off[P.col0 + c];
+    if (c == 0) {
+      const int q = t / kRedChunk;
+      item0[q] = P.item0; nl[q] = P.nl; ncol[q] = P.ncol; poff[q] = P.poff;
+    }
   }
   __syncthreads();
-  for (int lev = threadIdx.x; lev < nl; lev += blockDim.x)
-    v.partial[R.poff + lev] = ordered_sum<kRedChunk>(R.ncol, [&](int c) { return (double)base[coff[c] + lev]; });
+  if (nc == 1) {
+    for (int lev = t; lev < nl[0]; lev += blockDim.x)
+      v.partial[poff[0] + lev] = ordered_sum<kRedChunk>(ncol[0], [&](int c) { return (double)phi[coff[c] + lev]; });
+  } else {
+    int q = 0;
+    for (int i = 1; i < nc; ++i) q = item0[i] <= t ? i : q;
+    const int lev = t - item0[q];
+    if (lev < nl[q])
+      v.partial[poff[q] + lev] =
+          ordered_sum<kRedChunk>(ncol[q], [&](int c) { return (double)phi[coff[q * kRedChunk + c] + lev]; });
+  }
 }
 
 // Pass 2: one block per borehole; jacc (may be NULL) accumulates the outlet
@@ -380,12 +395,14 @@ hipError_t launch_line(const DevView& v, int prec, int mode, int dir, int lw, vo
 
 hipError_t launch_pipe_reduce(const DevView& v, int prec, const void* phi, hipStream_t st, double* jacc) {
   if (v.nbh == 0) return hipSuccess;
-  if (prec == 0)
-    hipLaunchKernelGGL((k_pipe_partial<double>), dim3(v.nrchunk), dim3(256), 0, st, v, (const double*)phi);
-  else
-    hipLaunchKernelGGL((k_pipe_partial<float>), dim3(v.nrchunk), dim3(256), 0, st, v, (const float*)phi);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (v.npblock > 0) {
+    if (prec == 0)
+      hipLaunchKernelGGL((k_pipe_partial<double>), dim3(v.npblock), dim3(256), 0, st, v, (const double*)phi);
+    else
+      hipLaunchKernelGGL((k_pipe_partial<float>), dim3(v.npblock), dim3(256), 0, st, v, (const float*)phi);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_pipe_final, dim3(v.nbh), dim3(kFinThreads), 0, st, v, jacc);
   return hipGetLastError();
 }

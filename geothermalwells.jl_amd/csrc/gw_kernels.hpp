@@ -21,6 +21,10 @@ inline int line_class(int n) {
 // Pipe columns per reduction chunk (one partial sum per level).  Fixed per
 // borehole, so the summation order does not depend on the batch.
 constexpr int kRedChunk = 16;
+// Partial-sum blocks: consecutive chunks with at most kPartItems levels in
+// all and at most kPartChunks chunks share a block (a chunk of more levels
+// has a block of its own and loops over them).
+constexpr int kPartItems = 256, kPartChunks = 16;
 constexpr int kAdvGroupMax = 8;  // most pipe columns of an advection group
 constexpr int kZcMax = 16;       // most column classes a scenario may have for the fused z-step kernels
 
@@ -59,6 +63,12 @@ struct DevView {
   const Dep* dep;
   const RedChunk* rchunk;
   int32_t nrchunk;
+  // the chunks again for the partial-sum kernel, and its blocks: block b
+  // sums chunks pblock[b] .. pblock[b+1]-1
+  const PartChunk* pchunk;
+  const int32_t* pblock;
+  int32_t npblock;
+  const int64_t* pcol_off;      // per pipe column: its level-0 offset within its scenario's state
   const RedChunk* agroup;       // advection groups (col0, ncol, bh, part; poff unused)
   int32_t nagroup;
   double* partial;

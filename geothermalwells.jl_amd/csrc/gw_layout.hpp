@@ -52,6 +52,16 @@ struct RedChunk {
   int32_t bh, part, col0, ncol, poff, pad;
 };
 
+// A reduction chunk as the partial-sum kernel reads it: the state offset of
+// its first summed level (scenario offset + the part's first level), its
+// columns [col0, col0+ncol) of the pipe list, its nl levels, where its sums
+// go (partial[poff + level]) and its first item in its block (a block packs
+// consecutive chunks, one thread per chunk level).
+struct PartChunk {
+  int64_t base;
+  int32_t col0, ncol, nl, poff, item0, pad;
+};
+
 // One pipe column: (i, j) of a borehole; inner-pipe columns of a borehole come
 // first, then its annulus columns (src/cache.jl:17-30 order).
 struct PipeCol {
