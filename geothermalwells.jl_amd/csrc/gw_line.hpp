@@ -790,16 +790,38 @@ template <> struct LineFamily<32> {
   using C9 = LineCfg<8, 40, 32>;
 };
 
-template <int LW>
+// FP32-state tiles of the 64-lane family: half the segment length on twice
+// the waves of the FP64 shapes up to the 416-element class (an FP32 line
+// takes half the registers, so the register budget no longer asks for long
+// segments; the FP64 recurrences' dependent chains halve).  The 640-element
+// class keeps its FP64 shape (sixteen-wave 40-element tiles measured slower).
+struct LineFamily64F32 {
+  using C0 = LineCfg<2, 24, 64>;
+  using C1 = LineCfg<4, 16, 64>;
+  using C2 = LineCfg<4, 24, 64>;
+  using C3 = LineCfg<8, 16, 64>;
+  using C4 = LineCfg<8, 20, 64>;
+  using C5 = LineCfg<8, 24, 64>;
+  using C6 = LineCfg<16, 16, 64>;
+  using C7 = LineCfg<16, 20, 64>;
+  using C8 = LineCfg<16, 26, 64>;
+  using C9 = LineCfg<8, 80, 64>;
+};
+// Tile shapes of a k-window family and state type.
+template <int LW, class TS> struct LineShapes { using F = LineFamily<LW>; };
+template <> struct LineShapes<64, float> { using F = LineFamily64F32; };
+
+template <class F>
 constexpr bool family_matches_classes() {
-  using F = LineFamily<LW>;
   return F::C0::NMAX == kLineClassMax[0] && F::C1::NMAX == kLineClassMax[1] && F::C2::NMAX == kLineClassMax[2] &&
          F::C3::NMAX == kLineClassMax[3] && F::C4::NMAX == kLineClassMax[4] && F::C5::NMAX == kLineClassMax[5] &&
          F::C6::NMAX == kLineClassMax[6] && F::C7::NMAX == kLineClassMax[7] && F::C8::NMAX == kLineClassMax[8] &&
          F::C9::NMAX == kLineClassMax[9];
 }
 static_assert(kLineClasses == 10, "one tile shape per line-length class");
-static_assert(family_matches_classes<64>() && family_matches_classes<32>(), "tile shapes follow the classes");
+static_assert(family_matches_classes<LineShapes<64, double>::F>() && family_matches_classes<LineShapes<32, double>::F>() &&
+                  family_matches_classes<LineShapes<64, float>::F>() && family_matches_classes<LineShapes<32, float>::F>(),
+              "tile shapes follow the classes");
 
 // One launch per non-empty line-length class, each with its own tile shape;
 // with side streams (ls) the largest launch stays on st and the others run
@@ -807,7 +829,7 @@ static_assert(family_matches_classes<64>() && family_matches_classes<32>(), "til
 template <int LW, class TS, bool SR = false>
 hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const void* inp, double c, double a1,
                        double a2, double a3, hipStream_t st, const LineStreams* ls) {
-  using F = LineFamily<LW>;
+  using F = typename LineShapes<LW, TS>::F;
   TS* out = static_cast<TS*>(outp);
   const TS* in = static_cast<const TS*>(inp);
   const bool halo = mode == LINE_ZY || mode == LINE_YZY || mode == LINE_YZY_KEEP;
