@@ -297,7 +297,7 @@ def bench_adjoint(gw, steps, seg, n_fd):
     lib = cache._lib
     u = u0.clone()
     torch.cuda.synchronize()
-    gw._abi.check(lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, dt, 8))  # tuning, graphs
+    gw._abi.check(lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, dt, 9))  # tuning, graphs or direct
     cache.sync()
     u.copy_(u0)
     torch.cuda.synchronize()
@@ -572,7 +572,7 @@ def main():
         "data": "synthetic (seeded inputs of the 3x3-array configuration; linear geothermal initial field)",
         "config": {"workload": "3x3 borehole array, 401x401x1002 cells, FP64, dt 300 s, per-well inlets",
                    "cells": cells, "boreholes": len(w["boreholes"]), "dt_s": dt,
-                   "steps_launch": cache.steps_launch},
+                   "steps_launch": cache.steps_launch, "line_launch": cache.line_launch},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,

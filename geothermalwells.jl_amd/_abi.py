@@ -98,6 +98,7 @@ SIGNATURES = {
     "gw_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "gw_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
     "gw_steps_mode": (ctypes.c_int, [ctypes.c_void_p]),
+    "gw_line_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "gw_line_op": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int32, ctypes.c_int32]),
     "gw_profile_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,

@@ -228,6 +228,23 @@ class Cache:
             _abi.check(m)
         return {0: None, 1: "graph", 2: "direct"}[m]
 
+    @property
+    def line_launch(self):
+        """The launch form of each line operator of the step pipeline ("zy",
+        "x", "yzy", "y"), as forced or as the first `gw_steps` call chose it:
+        the k-window width and which of the two-tiles LDS padding, whole halo
+        ranges and concurrent class launches it uses."""
+        out = {}
+        for op, name in enumerate(("zy", "x", "yzy", "y")):
+            f = self._lib.gw_line_launch(self._ctx, op)
+            if f < 0:
+                _abi.check(f)
+            out[name] = {"window": f & 96,  # 64 or 32
+                         "two_tiles": bool(f & _abi.GW_FLAG_LINE_TWO_TILES),
+                         "halo_whole": bool(f & _abi.GW_FLAG_HALO_WHOLE),
+                         "concurrent": bool(f & _abi.GW_FLAG_LINE_CONCURRENT)}
+        return out
+
 
 def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inlet_model, precision="f64",
                  stream=None, rock2_c3=DEFAULT_ROCK2_C3, line_window=None, line_two_tiles=False,
@@ -239,25 +256,30 @@ def create_cache(*, backend=None, gridx, gridy, gridz, materials, boreholes, inl
     of a class in one launch); None lets the first multi-step call time the
     variants and keep the fastest.  `steps_launch` ("graph" or "direct")
     forces how multi-step calls run their steps (None: timed on the first
-    call of 8 or more steps).  Results do not depend on these."""
+    call of 9 or more steps).  Results do not depend on these."""
     return Cache([_Scenario(gridx, gridy, gridz, materials, boreholes, inlet_model)], backend=backend,
                  precision=precision, stream=stream, rock2_c3=rock2_c3, line_window=line_window,
                  line_two_tiles=line_two_tiles, line_halo_whole=line_halo_whole, steps_launch=steps_launch)
 
 
 def create_batch_cache(scenarios, *, backend=None, precision="f64", stream=None, rock2_c3=DEFAULT_ROCK2_C3,
-                       line_window=None, line_concurrent=False, steps_launch=None, keep_line_classes=False):
+                       line_window=None, line_concurrent=False, steps_launch=None, keep_line_classes=False,
+                       line_halo_whole=False):
     """Many independent scenarios (each a dict with the create_cache keywords
     gridx, gridy, gridz, materials, boreholes, inlet_model) advanced together:
     every kernel launch covers all of them.  Line-length classes holding under
     5% of the batch's line cells run with the next larger class's tiles unless
     `keep_line_classes` (the results differ by rounding only: other segment
-    lengths split a line's recurrences elsewhere)."""
+    lengths split a line's recurrences elsewhere).  With a forced
+    `line_window`, `line_concurrent` runs an operator's class launches on side
+    streams and `line_halo_whole` launches each class's fused z-step tiles as
+    one range (same results); by default the first `gw_steps` call chooses
+    per operator (`line_launch`)."""
     sc = [_Scenario(d["gridx"], d["gridy"], d["gridz"], d["materials"], d["boreholes"], d["inlet_model"])
           for d in scenarios]
     return Cache(sc, backend=backend, precision=precision, stream=stream, rock2_c3=rock2_c3,
                  line_window=line_window, line_concurrent=line_concurrent, steps_launch=steps_launch,
-                 keep_line_classes=keep_line_classes)
+                 keep_line_classes=keep_line_classes, line_halo_whole=line_halo_whole)
 
 
 # ---------------------------------------------------------------------------

@@ -1385,12 +1385,14 @@ int tune_lines(gw_ctx& ctx, const CTab* tab, const void* phi, double dt) {
     float best = 0.f;
     int best_lw = ctx.lw_sel[mode][dir];
     const bool halo = mode == LINE_ZY || mode == LINE_YZY;
-    const int nvar = (halo ? 8 : 4) * (ctx.has_ls ? 2 : 1);
+    // batched contexts also time the class launches of an operator one after
+    // the other and concurrently on side streams (schedule 0, 1)
+    const int nb = halo ? 3 : 2;
+    const int nvar = (1 << nb) * (ctx.has_ls ? 2 : 1);
     for (int var0 = 0; var0 < nvar; ++var0) {
-      // bit 3 of the variant: concurrent class launches (batched contexts)
-      const int var = (var0 & 3) | (halo ? (var0 & 4) : 0) | ((var0 >> (halo ? 3 : 2)) ? 8 : 0);
+      const int var = var0 & ((1 << nb) - 1), sched = var0 >> nb;
       const int lw = (var & 1 ? 32 : 64) | (var & 2 ? kLwTwoTiles : 0) | (var & 4 ? kLwHaloWhole : 0) |
-                     (var & 8 ? kLwConcurrent : 0);
+                     (sched == 1 ? kLwConcurrent : 0);
       auto run = [&]() {
         return hip_status(launch_line(v, ctx.prec, mode, dir, lw, ctx.buf2, phi, dt / 2, dt, ctx.c3, ctx.stream, ctx.lsp()),
                           "line tune");
@@ -1458,8 +1460,10 @@ int gw_step(gw_ctx* ctx, void* phi, double t, double dt) {
 
 // Steps 2 .. nsteps-1 of a gw_steps call replay one captured graph per step
 // or launch the same kernels directly (identical results).  Unless a flag
-// forces one, the first call of at least 2 kStepTune + 2 steps times
-// kStepTune steps of each, after one of each, and keeps the faster: on large
+// forces one, the first call of at least 2 kStepTune + 3 steps times
+// kStepTune steps of each, after one of each (steps 1 .. 2 kStepTune + 2 of
+// the call; its last step, which ends with the implicit y alone, follows),
+// and keeps the faster: on large
 // fields the replays measured 1-7% slower per step than direct launches on
 // MI355X (ROCm 7), on small fields the launches' host cost can decide.
 constexpr int kStepTune = 3;
@@ -1486,7 +1490,7 @@ int gw_steps(gw_ctx* ctx, void* phi, double t, double dt, int32_t nsteps) {
   if (rc) return rc;
   // step k (1-based) works on B when k is odd, on phi when k is even
   const bool graph_ok = !ctx->host_inlet && nsteps > 2 && ctx->steps_mode != kStepsEager;
-  const bool tune = graph_ok && ctx->steps_mode == kStepsUndecided && nsteps >= 2 * kStepTune + 2;
+  const bool tune = graph_ok && ctx->steps_mode == kStepsUndecided && nsteps >= 2 * kStepTune + 3;
   if (graph_ok && (!ctx->graph_exec || ctx->graph_phi != phi || ctx->graph_dt != dt)) {
     if (ctx->graph_exec) { (void)hipGraphExecDestroy(ctx->graph_exec); ctx->graph_exec = nullptr; }
     if (ctx->graph_exec2) { (void)hipGraphExecDestroy(ctx->graph_exec2); ctx->graph_exec2 = nullptr; }
@@ -1713,6 +1717,14 @@ int gw_sync(gw_ctx* ctx) {
 int gw_steps_mode(gw_ctx* ctx) {
   if (!ctx) return fail(GW_EINVAL, "null argument");
   return ctx->steps_mode;
+}
+
+int gw_line_launch(gw_ctx* ctx, int32_t op) {
+  if (!ctx || op < 0 || op > 3) return fail(GW_EINVAL, "bad argument");
+  static const int ops[4][2] = {{LINE_ZY, 1}, {LINE_IMPLICIT_EXPLICIT, 0}, {LINE_YZY, 1}, {LINE_IMPLICIT, 1}};
+  const int lw = ctx->lw(ops[op][0], ops[op][1]);
+  return (lw & ~kLwFlags) | (lw & kLwTwoTiles ? GW_FLAG_LINE_TWO_TILES : 0) |
+         (lw & kLwHaloWhole ? GW_FLAG_HALO_WHOLE : 0) | (lw & kLwConcurrent ? GW_FLAG_LINE_CONCURRENT : 0);
 }
 
 void* gw_stream(gw_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }

@@ -56,7 +56,7 @@ enum { GW_F64 = 0, GW_F32_STATE = 1, GW_F32_STATE_SR = 2 };
  * coefficient-set count); GW_FLAG_LINE_CONCURRENT runs the launches of the
  * line-length classes of a batch concurrently on side streams.  gw_steps runs
  * the steps of a call by replaying one captured hipGraph per step or by
- * launching the same kernels directly; the first call of 8 or more steps times
+ * launching the same kernels directly; the first call of 9 or more steps times
  * both and keeps the faster, GW_FLAG_STEPS_GRAPH / _DIRECT force one.  In a
  * ragged batch, a line-length class holding under 5% of the batch's line
  * cells (in x and in y) runs with the next larger class's tile shape, one
@@ -278,9 +278,17 @@ int gw_sync(gw_ctx *ctx);
 void *gw_stream(gw_ctx *ctx);
 
 /* How gw_steps runs a call's steps: 0 not decided yet (graph replays until a
- * call of 8 or more steps has timed both), 1 graph replays, 2 direct launches;
+ * call of 9 or more steps has timed both), 1 graph replays, 2 direct launches;
  * < 0 on error. */
 int gw_steps_mode(gw_ctx *ctx);
+
+/* The launch form the context uses for line operator `op` (0: z-step +
+ * explicit y, 1: implicit + explicit x, 2: implicit y + z-step + explicit y,
+ * 3: implicit y): the k-window width (64 or 32) plus the
+ * GW_FLAG_LINE_TWO_TILES, GW_FLAG_HALO_WHOLE and GW_FLAG_LINE_CONCURRENT bits,
+ * as forced or as the first gw_steps call chose them; < 0 on error.  No
+ * counterpart in the reference (which has one launch form). */
+int gw_line_launch(gw_ctx *ctx, int32_t op);
 
 /* Measurement: nsteps full steps (no graph) with HIP events between the
  * kernels of the fused step pipeline, on the context's stream.  ms[] adds up:

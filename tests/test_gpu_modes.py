@@ -104,6 +104,52 @@ def test_batched_small_class_merged(prec):
         assert np.abs(Tm - Tk).max() <= 1e-4
 
 
+def test_batched_four_classes_launch_forms():
+    """A batch whose x and y lines fall in four line-length classes (75,
+    105/99, 131 and 169/167 points; two boreholes in one scenario), every
+    class kept: unforced, the first long call picks a launch form per line
+    operator (reported by gw_line_launch); forced to concurrent class
+    launches in either k-window width it gives the same field bit for bit
+    (the same tiles); per scenario it matches the oracle, the outlets per
+    borehole over the batch."""
+    torch = _torch()
+    g = H.gw()
+    ws = [H.small_case(n_fine=0.004, inlet="hx"),
+          H.small_case(nbh=2, n_fine=0.006, inlet="perwell"),
+          H.small_case(n_fine=0.002, layered=True),
+          H.small_case(n_fine=0.0015, h=45.0, nz_dz=(60.0, 2.0))]
+    n = 10
+    out = []
+    for flags in ({}, {"line_window": 64, "line_concurrent": True}, {"line_window": 32, "line_concurrent": True}):
+        cache = g.create_batch_cache([_kw(w) for w in ws], keep_line_classes=True, **flags)
+        u = cache.new_state()
+        for s, w in enumerate(ws):
+            u0 = H.oracle_for(w).initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+            cache.scenario_view(u, s).copy_(torch.from_numpy(u0))
+        g._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(u), 0.0, 60.0, n))
+        forms = cache.line_launch
+        print(flags, forms, flush=True)
+        assert all(f["window"] in (32, 64) for f in forms.values())
+        if flags:
+            assert all(f["window"] == flags["line_window"] and f["concurrent"] for f in forms.values())
+        out.append((u.cpu().numpy(), cache.T_outlet, cache))
+    u, T, cache = out[0]
+    same_width = [o for o, fl in zip(out[1:], (64, 32)) if all(f["window"] == fl for f in cache.line_launch.values())]
+    for u2, T2, _ in same_width:
+        assert np.array_equal(u, u2) and np.array_equal(T, T2)
+    b0 = 0
+    for s, w in enumerate(ws):
+        orc = H.oracle_for(w)
+        u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+        ref, hist, _ = orc.run(u0, 0.0, 60.0, n, cache.rock2_c3)
+        for uu, TT, cc in out:
+            got = cc.scenario_view(torch.from_numpy(uu), s).numpy()
+            assert H.rel_err(got, ref) <= 1e-10, (s, got.shape)
+            nb = len(w["boreholes"])
+            assert np.abs(np.asarray(TT[b0:b0 + nb]) - np.asarray(hist[-1][:nb])).max() <= 1e-6, s
+        b0 += len(w["boreholes"])
+
+
 def test_fp32_state_outlet_error():
     """FP32 field / stencil / advection with FP64 elimination and reductions:
     the outlet stays within 1e-3 K of the FP64 oracle."""

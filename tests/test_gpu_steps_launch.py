@@ -3,7 +3,7 @@ the same kernels.  Forced either way, and timed (the default), the results
 are identical bit for bit, for FP64, FP32 with stochastic rounding (its
 random numbers are keyed by the step counter the steps advance) and a ragged
 batch whose line classes launch concurrently on side streams; the timed
-choice is made by the first call of 8 or more steps and kept."""
+choice is made by the first call of 9 or more steps and kept."""
 import numpy as np
 import pytest
 
@@ -76,3 +76,23 @@ def test_batched_concurrent_graph_and_direct_identical():
         out.append((u.clone(), cache.T_outlet))
     assert torch.equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("n", [8, 9, 10])
+def test_first_timed_call_lengths(n):
+    """The timing call runs 2 + 2 * 3 ordinary steps before its last one: a
+    first call of 8 steps is too short for it and stays undecided, 9 steps is
+    the shortest that decides; either way the steps equal forced direct
+    launches bit for bit and match the oracle (an 8-step first call used to
+    time 8 ordinary steps and then run a ninth)."""
+    import torch
+    g = H.gw()
+    w = H.small_case(nbh=2, inlet="hx")
+    ua, Ta, ma, orc, u0 = _run(g, w, "f64", None, (n,))
+    ud, Td, _, _, _ = _run(g, w, "f64", "direct", (n,))
+    assert (ma is None) == (n < 9), ma
+    assert torch.equal(ua, ud)
+    assert np.array_equal(Ta, Td)
+    ref, hist, _ = orc.run(u0, 0.0, w["dt"], n, 1.0 / 16.0)
+    assert H.rel_err(ua.view(orc.shape).cpu().numpy(), ref) <= 1e-10
+    assert np.abs(Ta - hist[-1]).max() <= 1e-6
