@@ -174,9 +174,26 @@ def _init_state(gw, cache, ws, dtype):
     return u
 
 
+TUNE_STEPS = 9  # the shortest gw_steps call that times graph replays against direct launches
+
+
+def _tune(gw, cache, u, dt):
+    """The context's one-time choices (line-tile autotune, graph replays or
+    direct launches) made by an untimed call on a scratch copy of the state,
+    so the state itself gets exactly the warmup and timed steps asked for and
+    no timing falls inside the timed region whatever the warmup count."""
+    import torch
+    tmp = u.clone()
+    gw._abi.check(cache._lib.gw_steps(cache.ctx, cache.check_state(tmp), 0.0, dt, TUNE_STEPS))
+    cache.sync()
+    torch.cuda.synchronize()
+    del tmp
+
+
 def _timed_steps(gw, cache, u, dt, warmup, steps):
     import torch
     lib = cache._lib
+    _tune(gw, cache, u, dt)
     ptr = cache.check_state(u)
     gw._abi.check(lib.gw_steps(cache.ctx, ptr, 0.0, dt, warmup))
     cache.sync()
@@ -511,6 +528,7 @@ def main():
     ptr = cache.check_state(u)
     dt = w["dt"]
     torch.cuda.synchronize()
+    _tune(gw, cache, u, dt)
     gw._abi.check(lib.gw_steps(cache.ctx, ptr, 0.0, dt, args.warmup))
     cache.sync()
     torch.cuda.synchronize()
