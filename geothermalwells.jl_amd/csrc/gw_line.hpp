@@ -807,9 +807,18 @@ struct LineFamily64F32 {
   using C8 = LineCfg<16, 26, 64>;
   using C9 = LineCfg<8, 80, 64>;
 };
-// Tile shapes of a k-window family and state type.
-template <int LW, class TS> struct LineShapes { using F = LineFamily<LW>; };
-template <> struct LineShapes<64, float> { using F = LineFamily64F32; };
+// The stochastic-rounding FP32 kernels' 64-lane family: the 640-element class
+// on sixteen waves of 40 elements (the eight-wave 80-element tiles need all
+// 256 registers with the rounding's FP64 sums: one tile per CU, half the
+// waves).  5x5 array, 601-point lines: implicit + explicit x 1.83 -> 1.76-1.79
+// ms, step +1.2%; round to nearest keeps the eight-wave shape (0.5% faster).
+struct LineFamily64F32SR : LineFamily64F32 {
+  using C9 = LineCfg<16, 40, 64>;
+};
+// Tile shapes of a k-window family, state type and rounding.
+template <int LW, class TS, bool SR = false> struct LineShapes { using F = LineFamily<LW>; };
+template <> struct LineShapes<64, float, false> { using F = LineFamily64F32; };
+template <> struct LineShapes<64, float, true> { using F = LineFamily64F32SR; };
 
 template <class F>
 constexpr bool family_matches_classes() {
@@ -820,7 +829,8 @@ constexpr bool family_matches_classes() {
 }
 static_assert(kLineClasses == 10, "one tile shape per line-length class");
 static_assert(family_matches_classes<LineShapes<64, double>::F>() && family_matches_classes<LineShapes<32, double>::F>() &&
-                  family_matches_classes<LineShapes<64, float>::F>() && family_matches_classes<LineShapes<32, float>::F>(),
+                  family_matches_classes<LineShapes<64, float>::F>() && family_matches_classes<LineShapes<32, float>::F>() &&
+                  family_matches_classes<LineShapes<64, float, true>::F>(),
               "tile shapes follow the classes");
 
 // One launch per non-empty line-length class, each with its own tile shape;
@@ -829,7 +839,7 @@ static_assert(family_matches_classes<LineShapes<64, double>::F>() && family_matc
 template <int LW, class TS, bool SR = false>
 hipError_t line_family(const DevView& v, int mode, int dir, void* outp, const void* inp, double c, double a1,
                        double a2, double a3, hipStream_t st, const LineStreams* ls) {
-  using F = typename LineShapes<LW, TS>::F;
+  using F = typename LineShapes<LW, TS, SR>::F;
   TS* out = static_cast<TS*>(outp);
   const TS* in = static_cast<const TS*>(inp);
   const bool halo = mode == LINE_ZY || mode == LINE_YZY || mode == LINE_YZY_KEEP;
