@@ -36,8 +36,11 @@ typedef enum gw_status {
 } gw_status;
 
 /* State precision.  GW_F32_STATE: FP32 field, stencils and advection in FP32,
- * the Thomas recurrences and the outlet reductions in FP64 (each element
- * rounded to nearest).  GW_F32_STATE_SR: the same FP32 field, with every
+ * the Thomas recurrences and the outlet reductions in FP64.  An implicit solve
+ * keeps its line in FP32 registers between its two passes: each element is
+ * rounded to nearest twice (the forward pass's result bd_l y_l, then the
+ * solution); the recurrences themselves, their carries and the coefficients
+ * are FP64.  GW_F32_STATE_SR: the same FP32 field, with every
  * rounding of an updated value (the implicit solves' stores, the z-step's and
  * the explicit steps' results, computed as FP32 increments added in FP64)
  * stochastic, so sub-ulp changes are kept in expectation instead of lost: no
