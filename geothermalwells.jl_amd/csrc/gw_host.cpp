@@ -2216,6 +2216,42 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
     };
     double* A = MU0;  // mu_{m+1}, then mu_m
     double* Bf = MU1;
+    auto reverse = [&](int n) -> int {
+      for (int m = n - 1; m >= 0; --m) {
+        int r;
+        if ((r = adj_line(ADJ_Y, Bf, A, U[m + 1]))) return r;
+        if ((r = hip_status(launch_adj_pipe(v, ag, ap, Bf, seed, C.stream), "adjoint advection"))) return r;
+        if ((r = adj_line(ADJ_X, A, Bf, E[m]))) return r;
+        if ((r = hip_status(launch_adj_pipe(v, ag, ap, A, nullptr, C.stream), "adjoint advection"))) return r;
+        if ((r = hip_status(launch_adj_yz(v, ag, true, Bf, A, U[m], c, dt, C.c3, C.stream), "adjoint y + z")))
+          return r;
+        std::swap(A, Bf);
+      }
+      return GW_OK;
+    };
+    // The reverse steps of a full segment launch the same kernels on the same
+    // buffers in every segment (U, E, the two adjoint fields, the seed, the
+    // gradient accumulators), so with four or more segments they run as one
+    // graph replay per segment: one graph per starting adjoint field (two when
+    // seg is odd).  Graph or direct follows the context's gw_steps choice; when
+    // that is undecided the first two full segments are timed, one each way.
+    struct RevGraphs {
+      hipStream_t st;
+      hipGraphExec_t g[2] = {nullptr, nullptr};
+      hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+      ~RevGraphs() {
+        if (g[0] || g[1]) (void)hipStreamSynchronize(st);
+        for (auto x : g)
+          if (x) (void)hipGraphExecDestroy(x);
+        for (auto e : ev)
+          if (e) (void)hipEventDestroy(e);
+      }
+    } rg{C.stream};
+    int rev_mode = nseg >= 4 ? C.steps_mode : kStepsEager;
+    int timed = 0;  // full segments timed so far: the first direct, the second a graph replay
+    if (rev_mode == kStepsUndecided)
+      for (auto& e : rg.ev)
+        if ((rc = hip_status(hipEventCreate(&e), "event"))) return rc;
     for (int sg = nseg - 1; sg >= 0; --sg) {
       const int n = std::min(seg, nsteps - sg * seg);
       const int base = sg * seg;
@@ -2224,14 +2260,45 @@ int gw_adjoint(gw_ctx* ctx, const void* phi0, double t0, double dt, int32_t nste
       if ((rc = adj_recompute(C, tab, U.data(), E.data(), n, WB, WB2, t0 + (double)(base + 1) * dt, dt,
                               hist ? hist + (size_t)base * hstep : nullptr)))
         return rc;
-      for (int m = n - 1; m >= 0; --m) {
-        if ((rc = adj_line(ADJ_Y, Bf, A, U[m + 1]))) return rc;
-        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, Bf, seed, C.stream), "adjoint advection"))) return rc;
-        if ((rc = adj_line(ADJ_X, A, Bf, E[m]))) return rc;
-        if ((rc = hip_status(launch_adj_pipe(v, ag, ap, A, nullptr, C.stream), "adjoint advection"))) return rc;
-        if ((rc = hip_status(launch_adj_yz(v, ag, true, Bf, A, U[m], c, dt, C.c3, C.stream), "adjoint y + z")))
+      const bool full = n == seg && rev_mode != kStepsEager;
+      const bool timing = full && rev_mode == kStepsUndecided;
+      const bool graph = full && (rev_mode == kStepsGraph || timed == 1);
+      hipGraphExec_t& gx = rg.g[A == MU0 ? 0 : 1];
+      if (graph && !gx) {
+        double* const A0 = A;
+        double* const B0 = Bf;
+        hipGraph_t g = nullptr;
+        if ((rc = hip_status(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal), "begin capture")))
           return rc;
-        std::swap(A, Bf);
+        const int rc1 = reverse(n);
+        const hipError_t e2 = hipStreamEndCapture(C.stream, &g);
+        A = A0;
+        Bf = B0;
+        if (rc1 || e2 != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc1 ? rc1 : hip_status(e2, "end capture");
+        }
+        rc = hip_status(hipGraphInstantiate(&gx, g, nullptr, nullptr, 0), "graph instantiate");
+        (void)hipGraphDestroy(g);
+        if (rc) return rc;
+      }
+      if (timing && (rc = hip_status(hipEventRecord(rg.ev[2 * timed], C.stream), "event"))) return rc;
+      if (graph) {
+        if ((rc = hip_status(hipGraphLaunch(gx, C.stream), "graph launch"))) return rc;
+        if (n & 1) std::swap(A, Bf);
+      } else if ((rc = reverse(n))) {
+        return rc;
+      }
+      if (timing) {
+        if ((rc = hip_status(hipEventRecord(rg.ev[2 * timed + 1], C.stream), "event"))) return rc;
+        if (++timed == 2) {
+          float direct = 0.f, replay = 0.f;
+          if ((rc = hip_status(hipEventSynchronize(rg.ev[3]), "event sync")) ||
+              (rc = hip_status(hipEventElapsedTime(&direct, rg.ev[0], rg.ev[1]), "event time")) ||
+              (rc = hip_status(hipEventElapsedTime(&replay, rg.ev[2], rg.ev[3]), "event time")))
+            return rc;
+          rev_mode = replay < direct ? kStepsGraph : kStepsEager;
+        }
       }
     }
     if (dJ_du0 && (rc = hip_status(launch_mul_w(v, ag, static_cast<double*>(dJ_du0), A, C.stream), "W mu")))

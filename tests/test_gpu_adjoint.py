@@ -92,6 +92,35 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     assert np.abs(a[2] - b[2]).max() <= 1e-10 * np.abs(a[2]).max()
 
 
+@pytest.mark.parametrize("seg", [2, 3])
+def test_reverse_graph_replays_equal_direct_launches(seg):
+    """Four or more segments: the reverse steps of each full segment run as a
+    graph replay (one graph per starting adjoint field; two when the segment
+    length is odd), as direct launches, or timed one of each when the
+    context has not chosen.  All three give the gradient of one segment."""
+    w = H.small_case(nbh=2, inlet="perwell", layered=True)
+    g = H.gw()
+    orc = H.oracle_for(w)
+    u0 = orc.initial_condition(w["ic"]["T_surface"], w["ic"]["gradient"])
+    torch = _torch()
+    out = {}
+    for launch in ("graph", "direct", None):
+        cache = g.create_cache(steps_launch=launch, **_kw(w))
+        d = torch.from_numpy(u0).cuda().reshape(-1).contiguous()
+        print(f"adjoint reverse launch {launch}, seg {seg}", flush=True)
+        out[launch] = g.outlet_gradient(cache, d, 0.0, 60.0, 4 * seg + 1, checkpoint_every=seg)
+        if launch is None:
+            one = g.outlet_gradient(cache, d, 0.0, 60.0, 4 * seg + 1, checkpoint_every=4 * seg + 1)
+    for launch in ("graph", None):
+        a, b = out[launch], out["direct"]
+        assert a[0] == b[0]
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), launch
+    a = out["direct"]
+    assert abs(a[0] - one[0]) <= 1e-12 * abs(a[0])
+    assert np.abs(a[1] - one[1]).max() <= 1e-10 * np.abs(a[1]).max()
+    assert np.abs(a[2] - one[2]).max() <= 1e-10 * np.abs(a[2]).max()
+
+
 def test_adjoint_rejects_unsupported_configurations():
     torch = _torch()
     g = H.gw()
